@@ -1,0 +1,215 @@
+"""ecogen_amd: MI355X-native explicit finite-volume step of ECOGEN (Cartesian).
+
+Python host binding of ``libecogen_amd.so`` (C ABI in ``include/ecogen_amd.h``).
+The library is required: every entry point raises if it is missing, there is
+no CPU fallback.
+
+A *case* is a plain dict (see :mod:`ecogen_amd.case`) holding what ECOGEN's
+XML inputs describe for the hot path: mesh, flow model, EOS per phase,
+order/limiters, boundary conditions, CFL and the initial-condition domains.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libecogen_amd.so")
+_LIB = None
+
+MAX_PHASES = 4
+MODELS = {"euler": 0, "pueq": 1, "ueq": 2}
+EOS_TYPES = {"IG": 0, "SG": 1}
+LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5}
+BCS = {"nonreflecting": 0, "wall": 1, "symmetry": 1}
+DOMAINS = {"entire": 0, "halfspace": 1, "disc": 2, "sphere": 3, "rectangle": 4, "cuboid": 5}
+PRECISIONS = {"fp64": 0, "fp32": 1, "fp32_learned": 2}
+
+
+class EcgEos(C.Structure):
+    _fields_ = [("type", C.c_int), ("gamma", C.c_double), ("pinf", C.c_double), ("cv", C.c_double),
+                ("eref", C.c_double)]
+
+
+class EcgConfig(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+                ("model", C.c_int), ("nph", C.c_int), ("eos", EcgEos * MAX_PHASES), ("order", C.c_int),
+                ("lim", C.c_int * 4), ("alpha_null", C.c_int), ("relax_p", C.c_int), ("bc", C.c_int * 6),
+                ("cfl", C.c_double), ("precision", C.c_int)]
+
+
+class EcgDomain(C.Structure):
+    _fields_ = [("kind", C.c_int), ("axis", C.c_int), ("axis2", C.c_int), ("dir", C.c_int),
+                ("origin", C.c_double), ("center", C.c_double * 3), ("radius", C.c_double),
+                ("lo", C.c_double * 3), ("len", C.c_double * 3),
+                ("alpha", C.c_double * MAX_PHASES), ("rho", C.c_double * MAX_PHASES),
+                ("p", C.c_double * MAX_PHASES), ("vel", C.c_double * 3)]
+
+
+class EcgStepStats(C.Structure):
+    _fields_ = [("t", C.c_double), ("dt_next", C.c_double), ("dt_max", C.c_double), ("iteration", C.c_int64),
+                ("newton_fail", C.c_int64), ("newton_iters", C.c_int64), ("relax_calls", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# Exported symbols and their ctypes signatures (checked by the CPU test-suite).
+_DP = C.POINTER(C.c_double)
+SIGNATURES = {
+    "ecg_last_error": (C.c_char_p, []),
+    "ecg_version": (C.c_char_p, []),
+    "ecg_device_count": (C.c_int, []),
+    "ecg_create": (C.c_void_p, [C.POINTER(EcgConfig)]),
+    "ecg_destroy": (None, [C.c_void_p]),
+    "ecg_nvar": (C.c_int, [C.c_void_p]),
+    "ecg_ncell": (C.c_int64, [C.c_void_p]),
+    "ecg_fill": (C.c_int, [C.c_void_p, C.POINTER(EcgDomain), C.c_int]),
+    "ecg_upload_state": (C.c_int, [C.c_void_p, C.POINTER(_DP), C.c_int]),
+    "ecg_download_state": (C.c_int, [C.c_void_p, C.POINTER(_DP), C.c_int]),
+    "ecg_step": (C.c_int, [C.c_void_p, C.c_double, _DP, C.POINTER(EcgStepStats)]),
+    "ecg_run": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(EcgStepStats)]),
+    "ecg_enqueue_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "ecg_synchronize": (C.c_int, [C.c_void_p, C.POINTER(EcgStepStats)]),
+    "ecg_reset_clock": (C.c_int, [C.c_void_p]),
+    "ecg_stream": (C.c_void_p, [C.c_void_p]),
+}
+
+
+def library(path=None):
+    """Load the HIP library (raises when it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise RuntimeError(f"{p} is missing: build it with `python -m ecogen_amd.build`")
+        L = C.CDLL(p)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class EcgError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc < 0:
+        raise EcgError(library().ecg_last_error().decode())
+    return rc
+
+
+def make_config(case):
+    c = EcgConfig()
+    c.nx, c.ny, c.nz = case["nx"], case["ny"], case["nz"]
+    c.lx, c.ly, c.lz = case["lx"], case["ly"], case["lz"]
+    c.model = MODELS[case["model"]]
+    c.nph = len(case["eos"])
+    for k, e in enumerate(case["eos"]):
+        c.eos[k] = EcgEos(EOS_TYPES[e["type"].upper()], float(e["gamma"]), float(e.get("pinf", 0.0)),
+                          float(e.get("cv", 0.0)), float(e.get("eref", 0.0)))
+    c.order = case.get("order", 1)
+    lims = case.get("limiters", ("none",) * 4)
+    for i in range(4):
+        c.lim[i] = LIMITERS[lims[i].lower()]
+    c.alpha_null = int(bool(case.get("alpha_null", False)))
+    c.relax_p = int(bool(case.get("relax_p", case["model"] == "pueq")))
+    for i, b in enumerate(case.get("bc", ("nonreflecting",) * 6)):
+        c.bc[i] = BCS[b.lower()]
+    c.cfl = case.get("cfl", 0.8)
+    c.precision = PRECISIONS[case.get("precision", "fp64")]
+    return c
+
+
+def make_domains(case):
+    doms = case["domains"]
+    arr = (EcgDomain * len(doms))()
+    for i, d in enumerate(doms):
+        o = arr[i]
+        o.kind = DOMAINS[d["kind"]]
+        o.axis = d.get("axis", 0)
+        o.axis2 = d.get("axis2", 1)
+        o.dir = d.get("dir", 1)
+        o.origin = d.get("origin", 0.0)
+        o.radius = d.get("radius", 0.0)
+        for a in range(3):
+            o.center[a] = d.get("center", (0.0, 0.0, 0.0))[a]
+            o.lo[a] = d.get("lo", (0.0, 0.0, 0.0))[a]
+            o.len[a] = d.get("len", (0.0, 0.0, 0.0))[a]
+            o.vel[a] = d.get("vel", (0.0, 0.0, 0.0))[a]
+        n = len(d["rho"])
+        for k in range(n):
+            o.alpha[k] = d.get("alpha", [1.0] * n)[k]
+            o.rho[k] = d["rho"][k]
+            o.p[k] = d["p"][k]
+    return arr
+
+
+class Solver:
+    """One case on the GPU: the Run object of the reference, reduced to its hot path."""
+
+    def __init__(self, case, fill=True):
+        self.L = library()
+        self.case = case
+        self.cfg = make_config(case)
+        self.h = self.L.ecg_create(C.byref(self.cfg))
+        if not self.h:
+            raise EcgError(self.L.ecg_last_error().decode())
+        self.nvar = self.L.ecg_nvar(self.h)
+        self.ncell = self.L.ecg_ncell(self.h)
+        self.stats = EcgStepStats()
+        if fill and "domains" in case:
+            doms = make_domains(case)
+            _check(self.L.ecg_fill(self.h, doms, len(doms)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ecg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ptrs(self, arr):
+        return (_DP * self.nvar)(*[arr[v].ctypes.data_as(_DP) for v in range(self.nvar)])
+
+    def get_state(self):
+        out = np.empty((self.nvar, self.ncell), dtype=np.float64)
+        _check(self.L.ecg_download_state(self.h, self._ptrs(out), self.nvar))
+        return out
+
+    def set_state(self, st):
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        if st.shape != (self.nvar, self.ncell):
+            raise ValueError(f"state must have shape {(self.nvar, self.ncell)}")
+        _check(self.L.ecg_upload_state(self.h, self._ptrs(st), self.nvar))
+
+    def step(self, dt):
+        dtm = np.zeros(1)
+        rc = _check(self.L.ecg_step(self.h, float(dt), dtm.ctypes.data_as(_DP), C.byref(self.stats)))
+        return float(dtm[0]), rc
+
+    def run(self, max_iter=0, t_end=0.0):
+        _check(self.L.ecg_run(self.h, int(max_iter), float(t_end), C.byref(self.stats)))
+        return self.stats.as_dict()
+
+    def reset_clock(self):
+        _check(self.L.ecg_reset_clock(self.h))
+
+    def enqueue(self, n):
+        _check(self.L.ecg_enqueue_steps(self.h, int(n)))
+
+    def synchronize(self):
+        _check(self.L.ecg_synchronize(self.h, C.byref(self.stats)))
+        return self.stats.as_dict()
+
+    @property
+    def stream(self):
+        return self.L.ecg_stream(self.h)

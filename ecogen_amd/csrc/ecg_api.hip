@@ -1,0 +1,490 @@
+// C ABI of the finite-volume step (include/ecogen_amd.h): context, device
+// memory, the device-side time loop and the state transfers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ecogen_amd.h"
+#include "ecg_kernels.hpp"
+
+namespace ecg {
+step_fn get_step_fn_euler(int nph, int dim);
+step_fn get_step_fn_pueq(int nph, int dim);
+step_fn get_step_fn_ueq(int nph, int dim);
+fill_fn get_fill_fn_euler(int nph, int dim);
+fill_fn get_fill_fn_pueq(int nph, int dim);
+fill_fn get_fill_fn_ueq(int nph, int dim);
+
+step_fn get_step_fn(int model, int nph, int dim, int /*order*/) {
+  if (model == M_EULER) return get_step_fn_euler(nph, dim);
+  if (model == M_PUEQ) return get_step_fn_pueq(nph, dim);
+  if (model == M_UEQ) return get_step_fn_ueq(nph, dim);
+  return nullptr;
+}
+
+fill_fn get_fill_fn(int model, int nph, int dim) {
+  if (model == M_EULER) return get_fill_fn_euler(nph, dim);
+  if (model == M_PUEQ) return get_fill_fn_pueq(nph, dim);
+  if (model == M_UEQ) return get_fill_fn_ueq(nph, dim);
+  return nullptr;
+}
+
+// Run::solver bookkeeping after a step (Run.cpp:417-486): t += dt, dt = cfl*dtMax.
+__global__ void k_clock(Clock* clk, double cfl) {
+  if (threadIdx.x != 0 || clk->done) return;
+  const double dtmax = __longlong_as_double((long long)clk->dtmax_bits);
+  clk->dt_max = dtmax;
+  clk->t += clk->dt;
+  clk->iteration += 1;
+  clk->dt_next = cfl * dtmax;
+  if (clk->max_iter > 0) {
+    if (clk->iteration >= clk->max_iter) clk->done = 1;
+  } else if (clk->t >= clk->t_end) {
+    clk->done = 1;
+  }
+  clk->dt = clk->dt_next;
+  clk->dtmax_bits = (uint64_t)__double_as_longlong(1.e10);
+}
+
+// Host-set clock: reset or explicit-dt steps.
+__global__ void k_clock_set(Clock* clk, Clock v) {
+  if (threadIdx.x == 0) *clk = v;
+}
+
+void launch_clock(Clock* clk, double cfl, hipStream_t s) { hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, clk, cfl); }
+
+}  // namespace ecg
+
+using namespace ecg;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) return fail(ECG_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct ecg_ctx {
+  ecg_config cfg;
+  Params P;
+  int D = 1, nvar = 0;
+  int64_t ncell = 0, pitch = 0;
+  hipStream_t stream = nullptr;
+  double* W[3] = {nullptr, nullptr, nullptr};  // two time levels + second-order predicted state
+  int cur = 0;                                 // W[cur] holds the current state
+  double* tables = nullptr;
+  Clock* clk = nullptr;                        // device clock
+  Counters* cnt = nullptr;
+  Clock* hclk = nullptr;                       // pinned host mirror
+  Counters* hcnt = nullptr;
+  step_fn step = nullptr;
+  fill_fn fill = nullptr;
+  int64_t pending = 0;                         // steps enqueued since the clock was last read
+  int cur_at_sync = 0;
+};
+
+extern "C" {
+
+const char* ecg_last_error(void) { return g_err.c_str(); }
+
+const char* ecg_version(void) { return "ecogen_amd 0.1 (gfx950)"; }
+
+int ecg_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+static bool valid_eos(const ecg_eos& e) {
+  return (e.type == ECG_EOS_IG || e.type == ECG_EOS_SG) && std::isfinite(e.gamma) && e.gamma > 1.0 &&
+         std::isfinite(e.pinf) && std::isfinite(e.eref);
+}
+
+static EosDev make_eos(const ecg_eos& e) {
+  EosDev q;
+  q.sg = (e.type == ECG_EOS_SG);
+  q.g = e.gamma;
+  q.pinf = q.sg ? e.pinf : 0.;
+  q.gm1 = e.gamma - 1.;
+  q.gpinf = e.gamma * q.pinf;
+  q.eref = e.eref;
+  q.mixg = q.sg ? e.gamma * q.pinf / (e.gamma - 1.) : 0.;
+  q.mixinv = 1. / (e.gamma - 1.);
+  q.pclamp = -(1. - 1.e-15) * q.pinf + 1.e-15;
+  return q;
+}
+
+static int check_config(const ecg_config* c) {
+  if (!c) return fail(ECG_E_ARG, "null config");
+  if (c->nx < 2) return fail(ECG_E_ARG, "Number of cells in the x-direction has to be superior to 1");
+  if (c->ny < 1 || c->nz < 1) return fail(ECG_E_ARG, "cell counts must be positive");
+  if (c->nz > 1 && c->ny <= 1)
+    return fail(ECG_E_ARG, "Number of cells in the z-direction can't be superior to 1 if number of cells in the "
+                           "y-direction is equal to 1");
+  if (!(c->lx > 0 && c->ly > 0 && c->lz > 0)) return fail(ECG_E_ARG, "domain lengths must be positive");
+  if (c->model < ECG_EULER || c->model > ECG_UEQ) return fail(ECG_E_ARG, "unknown model");
+  if (c->model == ECG_EULER && c->nph != 1) return fail(ECG_E_ARG, "Euler model has exactly one phase");
+  if (c->model != ECG_EULER && (c->nph < 2 || c->nph > 3))
+    return fail(ECG_E_UNSUPPORTED, "multiphase models are built for 2 or 3 phases");
+  for (int k = 0; k < c->nph; k++)
+    if (!valid_eos(c->eos[k])) return fail(ECG_E_ARG, "invalid EOS parameters for phase " + std::to_string(k));
+  if (c->order != 1 && c->order != 2) return fail(ECG_E_ARG, "order must be 1 or 2");
+  if (c->order == 2)
+    for (int i = 0; i < 4; i++)
+      if (c->lim[i] < ECG_LIM_MINMOD || c->lim[i] > ECG_LIM_MC)
+        return fail(ECG_E_UNSUPPORTED, "second order needs minmod/vanleer/vanalbada/superbee/mc limiters");
+  for (int i = 0; i < 6; i++) {
+    if (c->bc[i] != ECG_BC_NONREFLECTING && c->bc[i] != ECG_BC_WALL) return fail(ECG_E_ARG, "unknown boundary");
+    if (c->order == 2 && c->bc[i] == ECG_BC_WALL)
+      return fail(ECG_E_UNSUPPORTED, "second-order wall/symmetry boundaries (BoundCondWallO2Cartesian) are not built");
+  }
+  if (!(c->cfl > 0)) return fail(ECG_E_ARG, "CFL must be positive");
+  if (c->precision != ECG_FP64) return fail(ECG_E_UNSUPPORTED, "this build runs the FP64 path only");
+  return ECG_OK;
+}
+
+ecg_ctx* ecg_create(const ecg_config* cfg) {
+  if (check_config(cfg) != ECG_OK) return nullptr;
+  ecg_ctx* c = new ecg_ctx();
+  c->cfg = *cfg;
+  if (cfg->model == ECG_PUEQ) c->cfg.relax_p = 1;
+  if (cfg->model == ECG_EULER) c->cfg.relax_p = 0;
+  const ecg_config& g = c->cfg;
+  c->D = (g.nx > 1) + (g.ny > 1) + (g.nz > 1);
+  c->ncell = (int64_t)g.nx * g.ny * g.nz;
+  c->pitch = (c->ncell + 63) / 64 * 64;
+  c->nvar = (g.model == ECG_EULER) ? (2 + c->D + 1) : (3 * g.nph + c->D + 1);
+  c->step = get_step_fn(g.model, g.nph, c->D, g.order);
+  c->fill = get_fill_fn(g.model, g.nph, c->D);
+  if (!c->step || !c->fill) {
+    fail(ECG_E_UNSUPPORTED, "no kernel built for this model/phase/dimension combination");
+    delete c;
+    return nullptr;
+  }
+  Params& P = c->P;
+  std::memset(&P, 0, sizeof(P));
+  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz;
+  P.ncell = c->ncell;
+  P.pitch = c->pitch;
+  // MeshCartesian::meshStretching / initializeGeometrieMonoCpu (MeshCartesian.cpp:168-246)
+  const int n3[3] = {g.nx, g.ny, g.nz};
+  const double L3[3] = {g.lx, g.ly, g.lz};
+  std::vector<double> pos[3];
+  for (int a = 0; a < 3; a++) {
+    P.d[a] = L3[a] / (double)n3[a];
+    pos[a].resize(n3[a]);
+    for (int i = 0; i < n3[a]; i++) pos[a][i] = (i + 0.5) * P.d[a];
+  }
+  const double volume = P.d[0] * P.d[1] * P.d[2];
+  P.coef[0] = P.d[1] * P.d[2] / volume;
+  P.coef[1] = P.d[0] * P.d[2] / volume;
+  P.coef[2] = P.d[0] * P.d[1] / volume;
+  double lcfl = 1.e10;
+  if (g.nx != 1) lcfl = std::min(lcfl, P.d[0]);
+  if (g.ny != 1) lcfl = std::min(lcfl, P.d[1]);
+  if (g.nz != 1) lcfl = std::min(lcfl, P.d[2]);
+  if (c->D > 1) lcfl *= 0.6;
+  P.lcfl = lcfl;
+  P.eps = (g.model != ECG_EULER && g.alpha_null) ? 1.e-15 : 0.;  // Eos.cpp:88-97
+  P.cfl = g.cfl;
+  P.alpha_null = g.alpha_null;
+  P.relax = g.relax_p;
+  for (int i = 0; i < 6; i++) P.bc[i] = g.bc[i];
+  for (int i = 0; i < 4; i++) P.lim[i] = g.lim[i];
+  for (int k = 0; k < g.nph; k++) P.eos[k] = make_eos(g.eos[k]);
+
+  // Distance tables of the interior faces (Element.cpp:77-81, Face.cpp:52-56).
+  std::vector<double> tab;
+  size_t off[3][3];
+  for (int a = 0; a < 3; a++) {
+    const int nf = std::max(n3[a] - 1, 1);
+    off[a][0] = tab.size(); tab.resize(tab.size() + nf, 0.);
+    off[a][1] = tab.size(); tab.resize(tab.size() + nf, 0.);
+    off[a][2] = tab.size(); tab.resize(tab.size() + nf, 0.);
+    for (int m = 0; m + 1 < n3[a]; m++) {
+      const double x = pos[a][m] - pos[a][m + 1];
+      const double fp = pos[a][m] + 0.5 * P.d[a];
+      const double xl = fp - pos[a][m], xr = fp - pos[a][m + 1];
+      tab[off[a][0] + m] = std::sqrt(x * x + 0. * 0. + 0. * 0.);
+      tab[off[a][1] + m] = std::sqrt(xl * xl + 0. * 0. + 0. * 0.);
+      tab[off[a][2] + m] = std::sqrt(xr * xr + 0. * 0. + 0. * 0.);
+    }
+  }
+  bool ok = true;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && ok) {
+      ok = false;
+      fail(ECG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+  };
+  chk(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  const size_t bytes = sizeof(double) * (size_t)c->pitch * c->nvar;
+  const int nbuf = (g.order == 2) ? 3 : 2;
+  for (int b = 0; b < nbuf && ok; b++) chk(hipMalloc(&c->W[b], bytes), "hipMalloc(state)");
+  if (ok) chk(hipMalloc(&c->tables, sizeof(double) * tab.size()), "hipMalloc(tables)");
+  if (ok) chk(hipMemcpy(c->tables, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice), "hipMemcpy(tables)");
+  if (ok) chk(hipMalloc(&c->clk, sizeof(Clock)), "hipMalloc(clock)");
+  if (ok) chk(hipMalloc(&c->cnt, sizeof(Counters)), "hipMalloc(counters)");
+  if (ok) chk(hipHostMalloc(&c->hclk, sizeof(Clock), hipHostMallocDefault), "hipHostMalloc(clock)");
+  if (ok) chk(hipHostMalloc(&c->hcnt, sizeof(Counters), hipHostMallocDefault), "hipHostMalloc(counters)");
+  if (!ok) {
+    ecg_destroy(c);
+    return nullptr;
+  }
+  for (int a = 0; a < 3; a++) {
+    P.dcc[a] = c->tables + off[a][0];
+    P.dfl[a] = c->tables + off[a][1];
+    P.dfr[a] = c->tables + off[a][2];
+  }
+  // Zero the state and counters, reset the clock.
+  for (int b = 0; b < nbuf; b++) hipMemset(c->W[b], 0, bytes);
+  hipMemset(c->cnt, 0, sizeof(Counters));
+  if (ecg_reset_clock(c) != ECG_OK) {
+    ecg_destroy(c);
+    return nullptr;
+  }
+  hipStreamSynchronize(c->stream);
+  return c;
+}
+
+void ecg_destroy(ecg_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (int b = 0; b < 3; b++)
+    if (c->W[b]) hipFree(c->W[b]);
+  if (c->tables) hipFree(c->tables);
+  if (c->clk) hipFree(c->clk);
+  if (c->cnt) hipFree(c->cnt);
+  if (c->hclk) hipHostFree(c->hclk);
+  if (c->hcnt) hipHostFree(c->hcnt);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int ecg_nvar(const ecg_ctx* c) { return c ? c->nvar : ECG_E_ARG; }
+int64_t ecg_ncell(const ecg_ctx* c) { return c ? c->ncell : ECG_E_ARG; }
+void* ecg_stream(ecg_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+static Clock fresh_clock(double dt, int64_t max_iter, double t_end) {
+  Clock k;
+  std::memset(&k, 0, sizeof(k));
+  k.t = 0.;
+  k.dt = dt;
+  k.dt_next = dt;
+  k.dt_max = 1.e10;
+  k.t_end = t_end;
+  k.iteration = 0;
+  k.max_iter = max_iter;
+  double big = 1.e10;
+  std::memcpy(&k.dtmax_bits, &big, sizeof(big));
+  k.done = 0;
+  return k;
+}
+
+static int set_clock(ecg_ctx* c, const Clock& k) {
+  hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(64), 0, c->stream, c->clk, k);
+  HIPCHK(hipGetLastError());
+  return ECG_OK;
+}
+
+int ecg_reset_clock(ecg_ctx* c) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  c->pending = 0;
+  c->cur_at_sync = c->cur;
+  return set_clock(c, fresh_clock(1.e-15, 0, INFINITY));
+}
+
+int ecg_fill(ecg_ctx* c, const ecg_domain* doms, int ndom) {
+  if (!c || !doms || ndom < 1) return fail(ECG_E_ARG, "ecg_fill needs at least one domain");
+  std::vector<DomainDev> dd(ndom);
+  for (int q = 0; q < ndom; q++) {
+    const ecg_domain& s = doms[q];
+    DomainDev& d = dd[q];
+    std::memset(&d, 0, sizeof(d));
+    if (s.kind < ECG_DOM_ENTIRE || s.kind > ECG_DOM_CUBOID) return fail(ECG_E_ARG, "unknown domain kind");
+    if (s.axis < 0 || s.axis > 2 || s.axis2 < 0 || s.axis2 > 2) return fail(ECG_E_ARG, "domain axis out of range");
+    d.kind = s.kind; d.axis = s.axis; d.axis2 = s.axis2; d.dir = s.dir; d.origin = s.origin; d.radius = s.radius;
+    for (int a = 0; a < 3; a++) {
+      d.center[a] = s.center[a]; d.lo[a] = s.lo[a]; d.len[a] = s.len[a]; d.vel[a] = s.vel[a];
+      if (a >= c->D && s.vel[a] != 0.)
+        return fail(ECG_E_ARG, "velocity along an inactive axis must be zero (state keeps active axes only)");
+    }
+    for (int k = 0; k < ECG_MAX_PHASES; k++) { d.alpha[k] = s.alpha[k]; d.rho[k] = s.rho[k]; d.p[k] = s.p[k]; }
+  }
+  DomainDev* ddev = nullptr;
+  HIPCHK(hipMalloc(&ddev, sizeof(DomainDev) * ndom));
+  HIPCHK(hipMemcpyAsync(ddev, dd.data(), sizeof(DomainDev) * ndom, hipMemcpyHostToDevice, c->stream));
+  const int threads = 256;
+  const int blocks = (int)((c->ncell + threads - 1) / threads);
+  c->fill(c->P, ddev, ndom, c->W[c->cur], c->stream, blocks, threads);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(ddev));
+  return ECG_OK;
+}
+
+int ecg_upload_state(ecg_ctx* c, const double* const* soa, int nvars) {
+  if (!c || !soa) return fail(ECG_E_ARG, "null argument");
+  if (nvars != c->nvar) return fail(ECG_E_ARG, "expected " + std::to_string(c->nvar) + " state arrays");
+  for (int v = 0; v < nvars; v++) {
+    if (!soa[v]) return fail(ECG_E_ARG, "null state array");
+    HIPCHK(hipMemcpyAsync(c->W[c->cur] + (size_t)v * c->pitch, soa[v], sizeof(double) * c->ncell,
+                          hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return ECG_OK;
+}
+
+static int settle(ecg_ctx* c);
+
+int ecg_download_state(ecg_ctx* c, double* const* soa, int nvars) {
+  if (!c || !soa) return fail(ECG_E_ARG, "null argument");
+  if (nvars != c->nvar) return fail(ECG_E_ARG, "expected " + std::to_string(c->nvar) + " state arrays");
+  int rc = settle(c);
+  if (rc != ECG_OK) return rc;
+  for (int v = 0; v < nvars; v++) {
+    if (!soa[v]) return fail(ECG_E_ARG, "null state array");
+    HIPCHK(hipMemcpyAsync(soa[v], c->W[c->cur] + (size_t)v * c->pitch, sizeof(double) * c->ncell,
+                          hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return ECG_OK;
+}
+
+// Enqueue one step on the device clock.  Buffers ping-pong; a step the clock
+// marks done is skipped by every kernel, so the current buffer is settled from
+// the executed iteration count after synchronising.
+static int enqueue_one(ecg_ctx* c) {
+  const int threads = 256;
+  const int blocks = (int)((c->ncell + threads - 1) / threads);
+  const int nxt = 1 - c->cur;
+  StepLaunch L;
+  if (c->cfg.order == 1) {
+    L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[nxt]; L.scale_half = 0; L.stage = 0;
+    c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads);
+  } else {
+    L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[2]; L.scale_half = 1; L.stage = 1;
+    c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads);
+    L.wn = c->W[c->cur]; L.wsrc = c->W[2]; L.wout = c->W[nxt]; L.scale_half = 0; L.stage = 2;
+    c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads);
+  }
+  launch_clock(c->clk, c->cfg.cfl, c->stream);
+  HIPCHK(hipGetLastError());
+  c->cur = nxt;
+  c->pending += 1;
+  return ECG_OK;
+}
+
+static int read_clock(ecg_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->hclk, c->clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return ECG_OK;
+}
+
+// Resolve which buffer holds the state after the enqueued steps.
+static int settle(ecg_ctx* c) {
+  if (c->pending == 0) return ECG_OK;
+  const int64_t before = c->hclk->iteration;
+  int rc = read_clock(c);
+  if (rc != ECG_OK) return rc;
+  (void)before;
+  c->pending = 0;
+  c->cur_at_sync = c->cur;
+  return ECG_OK;
+}
+
+static void fill_stats(ecg_ctx* c, ecg_step_stats* st) {
+  if (!st) return;
+  st->t = c->hclk->t;
+  st->dt_next = c->hclk->dt_next;
+  st->dt_max = c->hclk->dt_max;
+  st->iteration = c->hclk->iteration;
+  st->newton_fail = (int64_t)c->hcnt->newton_fail;
+  st->newton_iters = (int64_t)c->hcnt->newton_iters;
+  st->relax_calls = (int64_t)c->hcnt->relax_calls;
+}
+
+int ecg_step(ecg_ctx* c, double dt, double* dtmax_out, ecg_step_stats* st) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(ECG_E_ARG, "dt must be positive and finite");
+  int rc = read_clock(c);
+  if (rc != ECG_OK) return rc;
+  Clock k = *c->hclk;
+  const int64_t fails_before = (int64_t)c->hcnt->newton_fail;
+  k.dt = dt;
+  k.done = 0;
+  k.max_iter = 0;
+  k.t_end = INFINITY;
+  double big = 1.e10;
+  std::memcpy(&k.dtmax_bits, &big, sizeof(big));
+  rc = set_clock(c, k);
+  if (rc != ECG_OK) return rc;
+  rc = enqueue_one(c);
+  if (rc != ECG_OK) return rc;
+  rc = read_clock(c);
+  if (rc != ECG_OK) return rc;
+  c->pending = 0;
+  if (dtmax_out) *dtmax_out = c->hclk->dt_max;
+  fill_stats(c, st);
+  const int64_t nfail = (int64_t)c->hcnt->newton_fail - fails_before;
+  return nfail > 0 ? (int)nfail : ECG_OK;
+}
+
+int ecg_run(ecg_ctx* c, int64_t max_iter, double t_end, ecg_step_stats* st) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  if (max_iter <= 0 && !(t_end > 0)) return fail(ECG_E_ARG, "need max_iter > 0 or t_end > 0");
+  const int start = c->cur;
+  int rc = set_clock(c, fresh_clock(1.e-15, max_iter, max_iter > 0 ? INFINITY : t_end));
+  if (rc != ECG_OK) return rc;
+  int64_t enq = 0;
+  int batch = 8;
+  for (;;) {
+    for (int b = 0; b < batch; b++) {
+      rc = enqueue_one(c);
+      if (rc != ECG_OK) return rc;
+      enq++;
+      if (max_iter > 0 && enq >= max_iter) break;
+    }
+    rc = read_clock(c);
+    if (rc != ECG_OK) return rc;
+    if (c->hclk->done) break;
+    if (batch < 256) batch *= 2;
+  }
+  // The executed steps decide which buffer holds the final state.
+  const int64_t done_steps = c->hclk->iteration;
+  c->cur = (int)((start + done_steps) % 2);
+  c->pending = 0;
+  fill_stats(c, st);
+  return ECG_OK;
+}
+
+int ecg_enqueue_steps(ecg_ctx* c, int n) {
+  if (!c || n < 0) return fail(ECG_E_ARG, "bad arguments");
+  for (int i = 0; i < n; i++) {
+    int rc = enqueue_one(c);
+    if (rc != ECG_OK) return rc;
+  }
+  return ECG_OK;
+}
+
+int ecg_synchronize(ecg_ctx* c, ecg_step_stats* st) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  int rc = read_clock(c);
+  if (rc != ECG_OK) return rc;
+  c->pending = 0;
+  fill_stats(c, st);
+  return ECG_OK;
+}
+
+}  // extern "C"

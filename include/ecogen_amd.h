@@ -1,0 +1,149 @@
+/*
+ * ecogen_amd -- C ABI of the MI355X (gfx950) finite-volume step.
+ *
+ * Drop-in boundary for ECOGEN's explicit Cartesian step.  The reference calls
+ * its physics through virtual C++ interfaces at face/cell granularity:
+ *   Model::solveRiemannIntern / solveRiemannWall       src/Models/Model.h:109,127
+ *   Flux::{addFlux,subtractFlux,buildCons,buildPrim,
+ *          addNonCons,schemeCorrection,correctionEnergy} src/Models/Flux.h:48-100
+ *   Relaxation::relaxation                              src/Relaxations/Relaxation.h:58
+ *   Eos::compute*                                       src/Eos/Eos.h:104-164
+ *   Limiter::limiteSlope                                src/Order2/Limiter.h:43
+ * driven by Run::integrationProcedure / solveHyperbolic[O2] / solveRelaxations
+ * (src/Run.cpp:502-804).  Per-face virtual calls cannot cross a device
+ * boundary, so this ABI works at sweep/step granularity: one call advances the
+ * whole mesh by one reference time step (faces, cells, relaxation, energy
+ * correction, CFL reduction), on the context's HIP stream.
+ *
+ * Plain pointers and sizes only.  Host buffers are SoA double arrays of the
+ * persistent primitive set, variable-major:
+ *   multiphase (PUEQ/UEQ): [alpha_k, rho_k, p_k] for k = 0..N-1, then the
+ *                          velocity components of the active axes, then the
+ *                          mixture total energy E          (3N + D + 1 arrays)
+ *   Euler:                 rho, p, velocity (active axes), E (D + 3 arrays)
+ * Cells are indexed i + j*nx + k*nx*ny (MeshCartesian.cpp:141-147).
+ *
+ * Error policy: functions returning int give 0 on success, a negative ECG_E_*
+ * code on failure (message in ecg_last_error()).  ecg_step returns the number
+ * of relaxation fallbacks (the reference's warnings, RelaxationP.cpp:62-68)
+ * when positive.
+ */
+#ifndef ECOGEN_AMD_H
+#define ECOGEN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ECG_MAX_PHASES 4
+
+enum ecg_model { ECG_EULER = 0, ECG_PUEQ = 1, ECG_UEQ = 2 };
+enum ecg_eos_type { ECG_EOS_IG = 0, ECG_EOS_SG = 1 };
+enum ecg_limiter { ECG_LIM_NONE = 0, ECG_LIM_MINMOD = 1, ECG_LIM_VANLEER = 2, ECG_LIM_VANALBADA = 3,
+                   ECG_LIM_SUPERBEE = 4, ECG_LIM_MC = 5 };
+enum ecg_bc { ECG_BC_NONREFLECTING = 0, ECG_BC_WALL = 1 };
+/* Arithmetic of the hot path:
+ *   ECG_FP64          plain double precision, the parity path
+ *   ECG_FP32_RELAX64  fluxes, reconstruction and update in float with compensated
+ *                     accumulation; relaxation and EOS inversions in double
+ *   ECG_FP32_LEARNED  as ECG_FP32_RELAX64 with the Newton warm start predicted by the MLP */
+enum ecg_precision { ECG_FP64 = 0, ECG_FP32_RELAX64 = 1, ECG_FP32_LEARNED = 2 };
+enum ecg_domain_kind { ECG_DOM_ENTIRE = 0, ECG_DOM_HALFSPACE = 1, ECG_DOM_DISC = 2, ECG_DOM_SPHERE = 3,
+                       ECG_DOM_RECTANGLE = 4, ECG_DOM_CUBOID = 5 };
+
+enum {
+  ECG_OK = 0,
+  ECG_E_ARG = -1,       /* invalid argument or configuration */
+  ECG_E_HIP = -2,       /* HIP runtime error */
+  ECG_E_STATE = -3,     /* invalid / non-finite state */
+  ECG_E_UNSUPPORTED = -4
+};
+
+/* Eos::assignParametersEos (EosSG.cpp:48-57, EosIG.cpp:47-55) */
+typedef struct {
+  int type;               /* ecg_eos_type */
+  double gamma, pinf, cv, eref;
+} ecg_eos;
+
+/* What Input::readInputXML assembles for the hot path (Input.cpp:77-1193). */
+typedef struct {
+  int nx, ny, nz;         /* cells per axis; nx > 1, nz > 1 requires ny > 1 (Input.cpp:459-467) */
+  double lx, ly, lz;      /* domain lengths */
+  int model;              /* ecg_model */
+  int nph;                /* phases (1 for Euler) */
+  ecg_eos eos[ECG_MAX_PHASES];
+  int order;              /* 1: Godunov, 2: MUSCL + explicit midpoint */
+  int lim[4];             /* global, interface, global volume-fraction, interface volume-fraction */
+  int alpha_null;         /* flowModel alphaNull attribute */
+  int relax_p;            /* <relaxation type="P" speed="infinite"/> (forced for PUEQ) */
+  int bc[6];              /* X-, X+, Y-, Y+, Z-, Z+ (ecg_bc) */
+  double cfl;
+  int precision;          /* ecg_precision */
+} ecg_config;
+
+/* One initial-condition domain with its state (GeometricalDomain + Phase/Mixture states). */
+typedef struct {
+  int kind;               /* ecg_domain_kind */
+  int axis, axis2, dir;   /* half-space axis/direction, disc axes */
+  double origin;          /* half-space origin */
+  double center[3];
+  double radius;
+  double lo[3], len[3];   /* rectangle / cuboid lower corner and lengths */
+  double alpha[ECG_MAX_PHASES], rho[ECG_MAX_PHASES], p[ECG_MAX_PHASES];
+  double vel[3];
+} ecg_domain;
+
+typedef struct {
+  double t;               /* physical time after the last step */
+  double dt_next;         /* cfl * dtMax of the last step (Run.cpp:439) */
+  double dt_max;          /* min over the last step's Riemann calls of lCFL/|s| */
+  int64_t iteration;
+  int64_t newton_fail;    /* relaxations that fell back to the interface pressure (cumulative) */
+  int64_t newton_iters;   /* total Newton iterations (cumulative) */
+  int64_t relax_calls;    /* relaxed cells (cumulative) */
+} ecg_step_stats;
+
+typedef struct ecg_ctx ecg_ctx;
+
+/* Library / device ------------------------------------------------------- */
+const char* ecg_last_error(void);
+const char* ecg_version(void);
+int ecg_device_count(void);
+
+/* Context lifetime (Run::Run / Run::initialize / Run::finalize). */
+ecg_ctx* ecg_create(const ecg_config* cfg);
+void ecg_destroy(ecg_ctx* ctx);
+int ecg_nvar(const ecg_ctx* ctx);
+int64_t ecg_ncell(const ecg_ctx* ctx);
+
+/* Initial condition: Cell::fill over the domains in order, then completeFulfillState
+ * (Run.cpp:128-140).  Runs on the device. */
+int ecg_fill(ecg_ctx* ctx, const ecg_domain* doms, int ndom);
+
+/* State transfer, host SoA double arrays (layout above). soa[v] points at ncell doubles. */
+int ecg_upload_state(ecg_ctx* ctx, const double* const* soa, int nvars);
+int ecg_download_state(ecg_ctx* ctx, double* const* soa, int nvars);
+
+/* One reference step with an explicit dt: the work of Run::integrationProcedure +
+ * advancingProcedure (Run.cpp:502-623).  Synchronous; *dtmax_out receives dtMax. */
+int ecg_step(ecg_ctx* ctx, double dt, double* dtmax_out, ecg_step_stats* st);
+
+/* Reference time loop (Run::solver, Run.cpp:385-498) with dt0 = 1e-15 and the lagged
+ * dt_{n+1} = cfl*dtMax_n kept on the device.  Stops after max_iter steps when
+ * max_iter > 0, else at the first step with t >= t_end.  Restarts the clock. */
+int ecg_run(ecg_ctx* ctx, int64_t max_iter, double t_end, ecg_step_stats* st);
+
+/* Enqueue n steps of the device-clocked loop without synchronising (benchmarking). */
+int ecg_enqueue_steps(ecg_ctx* ctx, int n);
+int ecg_synchronize(ecg_ctx* ctx, ecg_step_stats* st);
+int ecg_reset_clock(ecg_ctx* ctx);
+
+/* Stream the context runs on (a hipStream_t), for event timing by the caller. */
+void* ecg_stream(ecg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

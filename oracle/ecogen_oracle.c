@@ -1,0 +1,1356 @@
+/*
+ * ORACLE -- test infrastructure only (see ecogen_oracle.h).
+ *
+ * Every arithmetic expression below keeps the association order of the
+ * reference expression it restates (the reference is built as ISO C++11 on
+ * x86-64 without FMA, so no contraction: compile this file with
+ * -ffp-contract=off).  Citations are path:line relative to the reference
+ * source tree.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ecogen_oracle.h"
+
+#define MAXPH ORC_MAXPH
+
+/* std::min / std::max semantics (return the first argument on ties). */
+static double omin(double a, double b) { return (b < a) ? b : a; }
+static double omax(double a, double b) { return (a < b) ? b : a; }
+/* Tools.h:86-88 sign(). */
+static int isign(double v) { return (0. < v) - (v < 0.); }
+/* Tools::returnNonZeroValue, Tools.cpp:134-145. */
+static double nonzero(double a, double eps)
+{
+  if (a > eps || a < -eps) return a;
+  return (double)isign(a) * eps;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Equations of state: EosSG.cpp:66-257, EosIG.cpp:64-255                    */
+/* ------------------------------------------------------------------------ */
+
+double orc_eos_energy(const orc_eos* q, double eps, double rho, double p)
+{
+  /* EosSG.cpp:73-76 / EosIG.cpp:71-74 */
+  if (q->type == ORC_EOS_SG) return (p + q->gamma * q->pinf) / (q->gamma - 1.) / omax(rho, eps) + q->eref;
+  return p / (q->gamma - 1.) / omax(rho, eps) + q->eref;
+}
+
+double orc_eos_pressure(const orc_eos* q, double rho, double e)
+{
+  /* EosSG.cpp:80-83 / EosIG.cpp:78-81 */
+  if (q->type == ORC_EOS_SG) return (q->gamma - 1.) * rho * (e - q->eref) - q->gamma * q->pinf;
+  return (q->gamma - 1.) * rho * (e - q->eref);
+}
+
+double orc_eos_sound(const orc_eos* q, double eps, double rho, double p)
+{
+  /* EosSG.cpp:94-97 / EosIG.cpp:92-95 */
+  if (q->type == ORC_EOS_SG) return sqrt(q->gamma * (p + q->pinf) / omax(rho, eps));
+  return sqrt(q->gamma * p / omax(rho, eps));
+}
+
+double orc_eos_temperature(const orc_eos* q, double eps, double rho, double p)
+{
+  /* EosSG.cpp:66-69 / EosIG.cpp:64-67 */
+  if (q->type == ORC_EOS_SG) return (p + q->pinf) / (q->gamma - 1.) / omax(rho, eps) / q->cv;
+  return p / (q->gamma - 1.) / omax(rho, eps) / q->cv;
+}
+
+double orc_eos_p_isentropic(const orc_eos* q, double eps, double p0, double rho0, double rhof)
+{
+  /* EosSG.cpp:129-132 / EosIG.cpp:127-130 */
+  if (q->type == ORC_EOS_SG) return (p0 + q->pinf) * pow(rhof / omax(rho0, eps), q->gamma) - q->pinf;
+  return p0 * pow(rhof / omax(rho0, eps), q->gamma);
+}
+
+double orc_eos_rho_pfinal(const orc_eos* q, double eps, double p0, double rho0, double pf, double* drhodp)
+{
+  if (q->type == ORC_EOS_SG) {
+    /* EosSG.cpp:163-170 */
+    double num = (q->gamma) * (pf + q->pinf);
+    double denom = num + p0 - pf;
+    double rhof = rho0 * num / nonzero(denom, eps);
+    if (drhodp) *drhodp = rho0 * q->gamma * (p0 + q->pinf) / omax((denom * denom), eps);
+    return rhof;
+  }
+  /* EosIG.cpp:161-168 */
+  {
+    double num = (q->gamma) * pf;
+    double denom = num + p0 - pf;
+    double rhof = rho0 * num / omax(denom, eps);
+    if (drhodp) *drhodp = rho0 * q->gamma * p0 / omax((denom * denom), eps);
+    return rhof;
+  }
+}
+
+double orc_eos_rho_hugoniot(const orc_eos* q, double eps, double p0, double rho0, double pf)
+{
+  if (q->type == ORC_EOS_SG) {
+    /* EosSG.cpp:152-159 */
+    double num = (q->gamma + 1.) * (pf + q->pinf) + (q->gamma - 1.) * (p0 + q->pinf);
+    double denom = (q->gamma - 1.) * (pf + q->pinf) + (q->gamma + 1.) * (p0 + q->pinf);
+    return rho0 * num / omax(denom, eps);
+  }
+  /* EosIG.cpp:150-157 */
+  {
+    double num = (q->gamma + 1.) * pf + (q->gamma - 1.) * p0;
+    double denom = (q->gamma - 1.) * pf + (q->gamma + 1.) * p0;
+    return rho0 * num / omax(denom, eps);
+  }
+}
+
+double orc_eos_clamp_p(const orc_eos* q, double p)
+{
+  /* EosSG.cpp:253-256 / EosIG.cpp:251-254 (verifyAndModifyPressure) */
+  if (q->type == ORC_EOS_SG) {
+    if (p <= -(1. - 1.e-15) * q->pinf + 1.e-15) p = -(1. - 1.e-15) * q->pinf + 1.e-15;
+    return p;
+  }
+  if (p < 1.e-8) p = 1.e-8;
+  return p;
+}
+
+/* EosSG.cpp:208-214 / EosIG.cpp:206-212 (sendSpecialMixtureEos) */
+static void mix_eos(const orc_eos* q, double* gpg, double* eref, double* inv, double* cov)
+{
+  *gpg = (q->type == ORC_EOS_SG) ? q->gamma * q->pinf / (q->gamma - 1.) : 0.;
+  *eref = q->eref;
+  *inv = 1. / (q->gamma - 1.);
+  *cov = 0.;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Limiters: LimiterMinmod.cpp:46-56, LimiterVanLeer.cpp:43-49,             */
+/* LimiterVanAlbada.cpp:43-51, LimiterSuperBee.cpp:46-55, LimiterMC.cpp:43-53 */
+/* ------------------------------------------------------------------------ */
+
+double orc_limiter(int kind, double s1, double s2)
+{
+  double slope = 0., produit = s1 * s2;
+  switch (kind) {
+    case ORC_LIM_MINMOD:
+      if (produit > 1e-9) {
+        slope = omin(fabs(s1), fabs(s2));
+        if (s1 < 0) slope = -slope;
+      }
+      return slope;
+    case ORC_LIM_VANLEER: {
+      double sum = s1 + s2;
+      if ((fabs(sum) > 1e-6) && (produit > 1e-6)) slope = 2. * produit / sum;
+      return slope;
+    }
+    case ORC_LIM_VANALBADA: {
+      double sum = s1 + s2;
+      if ((fabs(s1) > 1e-6) && (fabs(s2) > 1e-6) && (fabs(sum) > 1e-6) && (produit > 1e-6))
+        slope = produit * sum / (s1 * s1 + s2 * s2);
+      return slope;
+    }
+    case ORC_LIM_SUPERBEE:
+      if (produit > 1e-6) slope = omax(omin(2. * fabs(s1), fabs(s2)), omin(fabs(s1), 2. * fabs(s2)));
+      if (s1 < 0.) slope = -slope;
+      return slope;
+    case ORC_LIM_MC:
+      if (produit > 1e-9) {
+        slope = omin(2. * fabs(s1), 2. * fabs(s2));
+        slope = omin(slope, 0.5 * (fabs(s1) + fabs(s2)));
+      }
+      if (s1 < 0.) slope = -slope;
+      return slope;
+    default:
+      return 0.;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cell state                                                               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  double a[MAXPH], r[MAXPH], p[MAXPH];   /* persistent phase primitives */
+  double e[MAXPH], c[MAXPH], Y[MAXPH];   /* derived phase quantities */
+  double u[3];                            /* velocity (global frame, or face frame inside a Riemann call) */
+  double rho, pm, em, cF;                 /* mixture density, pressure, internal energy, frozen sound speed */
+  double E;                               /* total energy (mixture, or the Euler phase) */
+} cst;
+
+typedef struct {
+  double alpha[MAXPH], mass[MAXPH], energ[MAXPH];
+  double mom[3];
+  double energM;                          /* mixture total energy (Euler: the only energy) */
+} cons_t;
+
+typedef struct {
+  cons_t f;
+  double sM, uStar;
+} flux_t;
+
+typedef struct {
+  double a[MAXPH], r[MAXPH], p[MAXPH];
+  double u[3];
+} slope_t;
+
+struct orc_ctx {
+  orc_config cfg;
+  int D, n;                 /* dimensions, phases */
+  long long ncell;
+  int nvar;
+  double eps;               /* epsilonAlphaNull, Eos.cpp:88-97 */
+  double d[3];              /* cell sizes */
+  double *pos[3];           /* cell-centre coordinates per axis */
+  double coef[3];           /* surface / volume per face axis */
+  double lcfl;
+  cst* W;                   /* state at time n */
+  cst* Ws;                  /* second-order predicted state */
+  cons_t* acc;              /* per-cell flux accumulators (m_cons) */
+  slope_t* slopes;          /* per interior face slopes (second order) */
+  long long nface_int[3];
+  long long face_off[3];
+};
+
+static const orc_eos* EOSK(const orc_ctx* c, int k) { return &c->cfg.eos[k]; }
+
+static double sqn(const double* u)
+{
+  /* Coord::squaredNorm, Coord.cpp:89-92 */
+  return u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+}
+
+/* PhaseUEq::extendedCalculusPhase (PhaseUEq.cpp:97-102) + MixUEq::computeMixtureVariables (MixUEq.cpp:136-161);
+   Euler: PhaseEuler::extendedCalculusPhase (PhaseEuler.cpp:105-111). */
+static void fulfill(const orc_ctx* c, cst* s)
+{
+  int k;
+  if (c->cfg.model == ORC_EULER) {
+    s->e[0] = orc_eos_energy(EOSK(c, 0), c->eps, s->r[0], s->p[0]);
+    s->c[0] = orc_eos_sound(EOSK(c, 0), c->eps, s->r[0], s->p[0]);
+    s->E = s->e[0] + 0.5 * sqn(s->u);
+    return;
+  }
+  for (k = 0; k < c->n; k++) {
+    s->e[k] = orc_eos_energy(EOSK(c, k), c->eps, s->r[k], s->p[k]);
+    s->c[k] = orc_eos_sound(EOSK(c, k), c->eps, s->r[k], s->p[k]);
+  }
+  s->rho = 0.;
+  s->pm = 0.;
+  for (k = 0; k < c->n; k++) {
+    s->rho += s->a[k] * s->r[k];
+    s->pm += s->a[k] * s->p[k];
+  }
+  for (k = 0; k < c->n; k++) s->Y[k] = s->a[k] * s->r[k] / omax(s->rho, c->eps);
+  s->em = 0.;
+  s->cF = 0.;
+  for (k = 0; k < c->n; k++) {
+    s->em += s->Y[k] * s->e[k];
+    s->cF += s->Y[k] * s->c[k] * s->c[k];
+  }
+  s->cF = sqrt(s->cF);
+}
+
+/* PhaseUEq::verifyAndCorrectPhase (PhaseUEq.cpp:355-370); PhaseEuler (PhaseEuler.cpp:395-399) */
+static void verify_correct(const orc_ctx* c, cst* s, int k)
+{
+  if (c->cfg.model == ORC_EULER) {
+    if (s->r[0] < 1.e-10) s->r[0] = 1.e-10;
+    s->p[0] = orc_eos_clamp_p(EOSK(c, 0), s->p[0]);
+    return;
+  }
+  if (c->eps > 1.e-20) {
+    if (s->a[k] < 0.) s->a[k] = 0.;
+    if (s->a[k] > 1.) s->a[k] = 1.;
+    if (s->r[k] <= 1.e-15) s->r[k] = 1.e-15;
+  } else {
+    if (s->a[k] < 1e-15) s->a[k] = 1e-14;
+    if (s->a[k] > 1. - 1e-15) s->a[k] = 1. - 1e-14;
+    if (s->r[k] < 1.e-15) s->r[k] = 1.e-14;
+  }
+  s->p[k] = orc_eos_clamp_p(EOSK(c, k), s->p[k]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Face frames: MeshCartesian.cpp:282, 315, 348, 387, 411, 440, 482, 511, 535 */
+/* ------------------------------------------------------------------------ */
+
+enum { F_XI = 0, F_YI, F_ZI, F_XM, F_XP, F_YM, F_YP, F_ZM, F_ZP };
+static const double FRAME[9][3][3] = {
+  /* normal,        tangent,        binormal */
+  {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},     /* X internal */
+  {{0, 1, 0}, {-1, 0, 0}, {0, 0, 1}},    /* Y internal */
+  {{0, 0, 1}, {1, 0, 0}, {0, 1, 0}},     /* Z internal */
+  {{-1, 0, 0}, {0, -1, 0}, {0, 0, 1}},   /* X- */
+  {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},     /* X+ */
+  {{0, -1, 0}, {1, 0, 0}, {0, 0, 1}},    /* Y- */
+  {{0, 1, 0}, {-1, 0, 0}, {0, 0, 1}},    /* Y+ */
+  {{0, 0, -1}, {-1, 0, 0}, {0, 1, 0}},   /* Z- */
+  {{0, 0, 1}, {1, 0, 0}, {0, 1, 0}},     /* Z+ */
+};
+
+/* Coord::localProjection, Coord.cpp:133-140 (scalar products, Coord.cpp:106-109) */
+static void local_proj(double* u, int fr)
+{
+  const double (*F)[3] = FRAME[fr];
+  double x = u[0] * F[0][0] + u[1] * F[0][1] + u[2] * F[0][2];
+  double y = u[0] * F[1][0] + u[1] * F[1][1] + u[2] * F[1][2];
+  double z = u[0] * F[2][0] + u[1] * F[2][1] + u[2] * F[2][2];
+  u[0] = x; u[1] = y; u[2] = z;
+}
+
+/* Coord::reverseProjection, Coord.cpp:144-151 */
+static void reverse_proj(double* v, int fr)
+{
+  const double (*F)[3] = FRAME[fr];
+  double x = F[0][0] * v[0] + F[1][0] * v[1] + F[2][0] * v[2];
+  double y = F[0][1] * v[0] + F[1][1] * v[1] + F[2][1] * v[2];
+  double z = F[0][2] * v[0] + F[1][2] * v[1] + F[2][2] * v[2];
+  v[0] = x; v[1] = y; v[2] = z;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Riemann solvers (face frame)                                             */
+/* ------------------------------------------------------------------------ */
+
+static void dt_update(double* dtMax, double s, double dx)
+{
+  /* ModEuler.cpp:136-137, ModUEq.cpp:133-134 (machRefMin = 1) */
+  if (fabs(s) > 1.e-3) *dtMax = omin(*dtMax, 1. * dx / fabs(s));
+}
+
+/* ModEuler::solveRiemannIntern, ModEuler.cpp:105-221 */
+static void riemann_euler(const cst* Lc, const cst* Rc, double dxL, double dxR, double* dtMax, flux_t* F)
+{
+  double uL = Lc->u[0], vL = Lc->u[1], wL = Lc->u[2], pL = Lc->p[0], rhoL = Lc->r[0], cL = Lc->c[0], EL = Lc->E;
+  double uR = Rc->u[0], vR = Rc->u[1], wR = Rc->u[2], pR = Rc->p[0], rhoR = Rc->r[0], cR = Rc->c[0], ER = Rc->E;
+  double sL = omin(uL - cL, uR - cR);
+  double sR = omax(uR + cR, uL + cL);
+  double mL, mR, sM;
+  dt_update(dtMax, sL, dxL);
+  dt_update(dtMax, sR, dxR);
+  mL = rhoL * (sL - uL);
+  mR = rhoR * (sR - uR);
+  sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  if (fabs(sM) < 1.e-8) sM = 0.;
+  memset(F, 0, sizeof(*F));
+  if (sL > 0.) {
+    F->f.mass[0] = rhoL * uL;
+    F->f.mom[0] = rhoL * uL * uL + pL;
+    F->f.mom[1] = rhoL * vL * uL;
+    F->f.mom[2] = rhoL * wL * uL;
+    F->f.energM = (rhoL * EL + pL) * uL;
+  } else if (sR < 0.) {
+    F->f.mass[0] = rhoR * uR;
+    F->f.mom[0] = rhoR * uR * uR + pR;
+    F->f.mom[1] = rhoR * vR * uR;
+    F->f.mom[2] = rhoR * wR * uR;
+    F->f.energM = (rhoR * ER + pR) * uR;
+  } else if (sM >= 0.) {
+    double pStar = mL * (sM - uL) + pL;
+    double rhoStar = mL / (sL - sM);
+    double Estar = EL + (sM - uL) * (sM + pL / mL);
+    F->f.mass[0] = rhoStar * sM;
+    F->f.mom[0] = rhoStar * sM * sM + pStar;
+    F->f.mom[1] = rhoStar * sM * vL;
+    F->f.mom[2] = rhoStar * sM * wL;
+    F->f.energM = (rhoStar * Estar + pStar) * sM;
+  } else {
+    double pStar = mR * (sM - uR) + pR;
+    double rhoStar = mR / (sR - sM);
+    double Estar = ER + (sM - uR) * (sM + pR / mR);
+    F->f.mass[0] = rhoStar * sM;
+    F->f.mom[0] = rhoStar * sM * sM + pStar;
+    F->f.mom[1] = rhoStar * sM * vR;
+    F->f.mom[2] = rhoStar * sM * wR;
+    F->f.energM = (rhoStar * Estar + pStar) * sM;
+  }
+  F->sM = sM;
+  F->uStar = sM;
+}
+
+/* ModEuler::solveRiemannWall, ModEuler.cpp:368-409 */
+static void wall_euler(const cst* Lc, double dxL, double* dtMax, flux_t* F)
+{
+  double uL = Lc->u[0], pL = Lc->p[0], rhoL = Lc->r[0], cL = Lc->c[0];
+  double sL = omin(uL - cL, -uL - cL);
+  double pStar;
+  dt_update(dtMax, sL, dxL);
+  pStar = rhoL * uL * (uL - sL) + pL;
+  memset(F, 0, sizeof(*F));
+  F->f.mom[0] = pStar;
+  F->sM = 0.;
+  F->uStar = 0.;
+}
+
+/* ModUEq::solveRiemannIntern, ModUEq.cpp:113-268 (shared by PRESSUREVELOCITYEQ) */
+static void riemann_ueq(const orc_ctx* c, const cst* Lc, const cst* Rc, double dxL, double dxR, double* dtMax,
+                        flux_t* F)
+{
+  int k, n = c->n;
+  double uL = Lc->u[0], cL = Lc->cF, pL = Lc->pm, rhoL = Lc->rho;
+  double uR = Rc->u[0], cR = Rc->cF, pR = Rc->pm, rhoR = Rc->rho;
+  double sL = omin(uL - cL, uR - cR);
+  double sR = omax(uR + cR, uL + cL);
+  double mL, mR, sM;
+  dt_update(dtMax, sL, dxL);
+  dt_update(dtMax, sR, dxR);
+  mL = rhoL * (sL - uL);
+  mR = rhoR * (sR - uR);
+  sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  if (fabs(sM) < 1.e-8) sM = 0.;
+  memset(F, 0, sizeof(*F));
+  if (sL >= 0.) {
+    double vitY = Lc->u[1], vitZ = Lc->u[2], totalEnergy = Lc->em + 0.5 * sqn(Lc->u);
+    for (k = 0; k < n; k++) {
+      F->f.alpha[k] = Lc->a[k] * sM;
+      F->f.mass[k] = Lc->a[k] * Lc->r[k] * uL;
+      F->f.energ[k] = Lc->a[k] * Lc->r[k] * Lc->e[k] * uL;
+    }
+    F->f.mom[0] = rhoL * uL * uL + pL;
+    F->f.mom[1] = rhoL * vitY * uL;
+    F->f.mom[2] = rhoL * vitZ * uL;
+    F->f.energM = (rhoL * totalEnergy + pL) * uL;
+    F->uStar = uL;
+  } else if (sR <= 0.) {
+    double vitY = Rc->u[1], vitZ = Rc->u[2], totalEnergy = Rc->em + 0.5 * sqn(Rc->u);
+    for (k = 0; k < n; k++) {
+      F->f.alpha[k] = Rc->a[k] * sM;
+      F->f.mass[k] = Rc->a[k] * Rc->r[k] * uR;
+      F->f.energ[k] = Rc->a[k] * Rc->r[k] * Rc->e[k] * uR;
+    }
+    F->f.mom[0] = rhoR * uR * uR + pR;
+    F->f.mom[1] = rhoR * vitY * uR;
+    F->f.mom[2] = rhoR * vitZ * uR;
+    F->f.energM = (rhoR * totalEnergy + pR) * uR;
+    F->uStar = uR;
+  } else {
+    const cst* S = (sM >= 0.) ? Lc : Rc;
+    double uS = (sM >= 0.) ? uL : uR, pS = (sM >= 0.) ? pL : pR, mS = (sM >= 0.) ? mL : mR;
+    double sS = (sM >= 0.) ? sL : sR;
+    double vitY = S->u[1], vitZ = S->u[2], totalEnergy = S->em + 0.5 * sqn(S->u);
+    double rhoStar = mS / (sS - sM);
+    double EStar = totalEnergy + (sM - uS) * (sM + pS / mS);
+    double pStar = mS * (sM - uS) + pS;
+    for (k = 0; k < n; k++) {
+      double mk = S->r[k] * (sS - uS);
+      double rks = mk / (sS - sM);
+      double pks = orc_eos_p_isentropic(EOSK(c, k), c->eps, S->p[k], S->r[k], rks);
+      double eks = orc_eos_energy(EOSK(c, k), c->eps, rks, pks);
+      F->f.alpha[k] = S->a[k] * sM;
+      F->f.mass[k] = S->a[k] * rks * sM;
+      F->f.energ[k] = S->a[k] * rks * eks * sM;
+    }
+    F->f.mom[0] = rhoStar * sM * sM + pStar;
+    F->f.mom[1] = rhoStar * vitY * sM;
+    F->f.mom[2] = rhoStar * vitZ * sM;
+    F->f.energM = (rhoStar * EStar + pStar) * sM;
+    F->uStar = sM;
+  }
+  F->sM = sM;
+}
+
+/* ModUEq::solveRiemannWall, ModUEq.cpp:452-493 */
+static void wall_ueq(const cst* Lc, double dxL, double* dtMax, flux_t* F)
+{
+  double uL = Lc->u[0], cL = Lc->cF, pL = Lc->pm, rhoL = Lc->rho;
+  double sL = omin(uL - cL, -uL - cL);
+  double pStar;
+  dt_update(dtMax, sL, dxL);
+  pStar = rhoL * (uL - sL) * uL + pL;
+  memset(F, 0, sizeof(*F));
+  F->f.mom[0] = pStar;
+  F->sM = 0.;
+  F->uStar = 0.;
+}
+
+static void riemann_intern(const orc_ctx* c, const cst* L, const cst* R, double* dtMax, flux_t* F)
+{
+  if (c->cfg.model == ORC_EULER) riemann_euler(L, R, c->lcfl, c->lcfl, dtMax, F);
+  else riemann_ueq(c, L, R, c->lcfl, c->lcfl, dtMax, F);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Flux accumulation: CellInterface.cpp:267-297, FluxUEq.cpp:61-84, 224-244, */
+/* FluxPUEq.cpp:44-65, FluxEuler.cpp:52-73                                   */
+/* ------------------------------------------------------------------------ */
+
+static void add_flux(const orc_ctx* c, cons_t* A, const flux_t* F, double coefA, const cst* cellN)
+{
+  int k;
+  if (c->cfg.model == ORC_EULER) {
+    A->mass[0] += coefA * F->f.mass[0];
+    A->mom[0] += F->f.mom[0] * coefA;
+    A->mom[1] += F->f.mom[1] * coefA;
+    A->mom[2] += F->f.mom[2] * coefA;
+    A->energM += coefA * F->f.energM;
+    return;
+  }
+  for (k = 0; k < c->n; k++) {
+    A->alpha[k] += coefA * F->f.alpha[k];
+    A->mass[k] += coefA * F->f.mass[k];
+    A->energ[k] += coefA * F->f.energ[k];
+  }
+  A->mom[0] += F->f.mom[0] * coefA;
+  A->mom[1] += F->f.mom[1] * coefA;
+  A->mom[2] += F->f.mom[2] * coefA;
+  A->energM += coefA * F->f.energM;
+  /* addNonCons uses the cell's own time-n phases (Cell::getPhase default type) */
+  for (k = 0; k < c->n; k++) {
+    A->alpha[k] += -coefA * cellN->a[k] * F->sM;
+    if (c->cfg.model == ORC_PUEQ) A->energ[k] += coefA * cellN->a[k] * cellN->p[k] * F->uStar;
+  }
+}
+
+static void sub_flux(const orc_ctx* c, cons_t* A, const flux_t* F, double coefA, const cst* cellN)
+{
+  int k;
+  if (c->cfg.model == ORC_EULER) {
+    A->mass[0] -= coefA * F->f.mass[0];
+    A->mom[0] -= F->f.mom[0] * coefA;
+    A->mom[1] -= F->f.mom[1] * coefA;
+    A->mom[2] -= F->f.mom[2] * coefA;
+    A->energM -= coefA * F->f.energM;
+    return;
+  }
+  for (k = 0; k < c->n; k++) {
+    A->alpha[k] -= coefA * F->f.alpha[k];
+    A->mass[k] -= coefA * F->f.mass[k];
+    A->energ[k] -= coefA * F->f.energ[k];
+  }
+  A->mom[0] -= F->f.mom[0] * coefA;
+  A->mom[1] -= F->f.mom[1] * coefA;
+  A->mom[2] -= F->f.mom[2] * coefA;
+  A->energM -= coefA * F->f.energM;
+  for (k = 0; k < c->n; k++) {
+    A->alpha[k] -= -coefA * cellN->a[k] * F->sM;
+    if (c->cfg.model == ORC_PUEQ) A->energ[k] -= coefA * cellN->a[k] * cellN->p[k] * F->uStar;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cell update                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* FluxUEq::buildCons (FluxUEq.cpp:147-157), FluxEuler::buildCons (FluxEuler.cpp:113-120) */
+static void build_cons(const orc_ctx* c, const cst* s, cons_t* U)
+{
+  int k;
+  memset(U, 0, sizeof(*U));
+  if (c->cfg.model == ORC_EULER) {
+    U->mass[0] = s->r[0];
+    U->mom[0] = s->u[0] * U->mass[0];
+    U->mom[1] = s->u[1] * U->mass[0];
+    U->mom[2] = s->u[2] * U->mass[0];
+    U->energM = U->mass[0] * s->E;
+    return;
+  }
+  for (k = 0; k < c->n; k++) {
+    U->alpha[k] = s->a[k];
+    U->mass[k] = s->a[k] * s->r[k];
+    U->energ[k] = s->a[k] * s->r[k] * orc_eos_energy(EOSK(c, k), c->eps, s->r[k], s->p[k]);
+  }
+  U->mom[0] = s->u[0] * s->rho;
+  U->mom[1] = s->u[1] * s->rho;
+  U->mom[2] = s->u[2] * s->rho;
+  U->energM = s->rho * s->E;
+}
+
+static void cons_scale(const orc_ctx* c, cons_t* A, double x)
+{
+  int k;
+  for (k = 0; k < c->n; k++) {
+    A->alpha[k] *= x;
+    A->mass[k] *= x;
+    A->energ[k] *= x;
+  }
+  A->mom[0] *= x; A->mom[1] *= x; A->mom[2] *= x;
+  A->energM *= x;
+}
+
+static void cons_add(const orc_ctx* c, cons_t* A, const cons_t* U)
+{
+  int k;
+  for (k = 0; k < c->n; k++) {
+    A->alpha[k] += 1. * U->alpha[k];
+    A->mass[k] += 1. * U->mass[k];
+    A->energ[k] += 1. * U->energ[k];
+  }
+  A->mom[0] += U->mom[0] * 1.; A->mom[1] += U->mom[1] * 1.; A->mom[2] += U->mom[2] * 1.;
+  A->energM += 1. * U->energM;
+}
+
+/* FluxUEq::schemeCorrection, FluxUEq.cpp:248-269 (VELOCITYEQ only: FluxPUEq.h overrides it empty) */
+static void scheme_correction(const orc_ctx* c, cons_t* A, const cons_t* Un, const cst* cellN)
+{
+  int k;
+  double rhoN = cellN->rho, rhoNp1 = 0., sumD = 0., mom[3], dRU2, dRIE, err;
+  for (k = 0; k < c->n; k++) {
+    rhoNp1 += A->mass[k] + Un->mass[k];
+    sumD += A->energ[k];
+  }
+  mom[0] = A->mom[0] + Un->mom[0];
+  mom[1] = A->mom[1] + Un->mom[1];
+  mom[2] = A->mom[2] + Un->mom[2];
+  dRU2 = sqn(mom) / rhoNp1 - sqn(Un->mom) / rhoN;
+  dRIE = A->energM - 0.5 * dRU2;
+  err = dRIE - sumD;
+  for (k = 0; k < c->n; k++) A->energ[k] += cellN->a[k] * err;
+}
+
+/* Cell::timeEvolution (Cell.cpp:286-299) with scale = dt, CellO2::predictionOrdre2 (CellO2.cpp:111-118) with 0.5*dt */
+static void time_evolution(const orc_ctx* c, cons_t* A, const cst* cellN, double scale)
+{
+  cons_t Un;
+  build_cons(c, cellN, &Un);
+  cons_scale(c, A, scale);
+  if (c->cfg.model == ORC_UEQ) scheme_correction(c, A, &Un, cellN);
+  cons_add(c, A, &Un);
+}
+
+/* FluxUEq::buildPrim (FluxUEq.cpp:161-207), FluxEuler::buildPrim (FluxEuler.cpp:124-150) */
+static void build_prim(const orc_ctx* c, cons_t* A, cst* s)
+{
+  int k, n = c->n;
+  double un = 0., rhoMel = 0.;
+  if (c->cfg.model == ORC_EULER) {
+    double e;
+    s->r[0] = A->mass[0];
+    s->u[0] = A->mom[0] / A->mass[0];
+    s->u[1] = A->mom[1] / A->mass[0];
+    s->u[2] = A->mom[2] / A->mass[0];
+    if (fabs(s->u[0]) < 1.e-8) s->u[0] = 0.;
+    if (fabs(s->u[1]) < 1.e-8) s->u[1] = 0.;
+    if (fabs(s->u[2]) < 1.e-8) s->u[2] = 0.;
+    s->E = A->energM / A->mass[0];
+    e = s->E - 0.5 * sqn(s->u);
+    s->e[0] = e;
+    s->p[0] = orc_eos_pressure(EOSK(c, 0), A->mass[0], e);
+    s->c[0] = orc_eos_sound(EOSK(c, 0), c->eps, A->mass[0], s->p[0]);
+    return;
+  }
+  if (c->eps > 1.e-20) {
+    for (k = 0; k < n; k++) {
+      if (A->alpha[k] < 0.) A->alpha[k] = 0.;
+      if (A->alpha[k] > 1.) A->alpha[k] = 1.;
+      un += A->alpha[k];
+    }
+  } else {
+    for (k = 0; k < n; k++) {
+      if (A->alpha[k] <= 1.e-15) A->alpha[k] = 1e-15;
+      if (A->alpha[k] >= 1. - 1.e-15) A->alpha[k] = 1.0 - 1e-15;
+      un += A->alpha[k];
+    }
+  }
+  for (k = 0; k < n; k++) A->alpha[k] /= un;
+  for (k = 0; k < n; k++) {
+    double ek;
+    rhoMel += A->mass[k];
+    s->a[k] = A->alpha[k];
+    s->r[k] = A->mass[k] / omax(A->alpha[k], c->eps);
+    ek = A->energ[k] / omax(A->mass[k], c->eps);
+    s->p[k] = orc_eos_pressure(EOSK(c, k), s->r[k], ek);
+    verify_correct(c, s, k);
+  }
+  s->u[0] = A->mom[0] / rhoMel;
+  s->u[1] = A->mom[1] / rhoMel;
+  s->u[2] = A->mom[2] / rhoMel;
+  if (fabs(s->u[0]) < 1.e-8) s->u[0] = 0.;
+  if (fabs(s->u[1]) < 1.e-8) s->u[1] = 0.;
+  if (fabs(s->u[2]) < 1.e-8) s->u[2] = 0.;
+  fulfill(c, s);
+  s->E = A->energM / rhoMel;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Relaxation: RelaxationPInfinite.cpp:43-98, RelaxationP.cpp:43-93          */
+/* ------------------------------------------------------------------------ */
+
+static int newton(int n, const orc_eos* eos, double eps, const double* ak, const double* rhok, const double* pk,
+                  double* pStar, double* akS, double* rhokS)
+{
+  double f = 0., df = 1., drhodp = 0., dalpha = 0.;
+  int it = 0, k;
+  do {
+    it++;
+    *pStar -= f / df;
+    for (k = 0; k < n; k++) *pStar = orc_eos_clamp_p(&eos[k], *pStar);
+    f = -1.;
+    df = 0.;
+    for (k = 0; k < n; k++) {
+      rhokS[k] = orc_eos_rho_pfinal(&eos[k], eps, pk[k], rhok[k], *pStar, &drhodp);
+      akS[k] = ak[k] * rhok[k] / rhokS[k];
+      dalpha = ak[k] * rhok[k] * drhodp / (rhokS[k] * rhokS[k]);
+      f += akS[k];
+      df -= dalpha;
+    }
+  } while (fabs(f) > 1e-10 && it < 100);
+  return it;
+}
+
+/* Returns Newton iterations (0 when the cell is gated out). */
+static int relax_state(int n, const orc_eos* eos, double eps, double* a, double* r, double* p, const double* csound,
+                       double* pout)
+{
+  double ak[MAXPH], pk[MAXPH], rhok[MAXPH], akS[MAXPH], rhokS[MAXPH], pStar = 0.;
+  int k, it;
+  if (eps > 1.e-20) {
+    for (k = 0; k < n; k++)
+      if (a[k] > (1. - 1.e-5)) return 0;
+  }
+  for (k = 0; k < n; k++) {
+    ak[k] = a[k];
+    pk[k] = p[k];
+    rhok[k] = r[k];
+    pStar += ak[k] * pk[k];
+  }
+  it = newton(n, eos, eps, ak, rhok, pk, &pStar, akS, rhokS);
+  if (it < 100) {
+    for (k = 0; k < n; k++) {
+      a[k] = akS[k];
+      r[k] = rhokS[k];
+      p[k] = pStar;
+    }
+    if (pout) *pout = pStar;
+  } else {
+    /* RelaxationP::computeInterfacePressure, RelaxationP.cpp:72-93 */
+    double pI = 0., sumZk = 0., sumZj, zk[MAXPH], drhodp;
+    int j;
+    for (k = 0; k < n; k++) zk[k] = r[k] * csound[k];
+    for (k = 0; k < n; k++) {
+      sumZk += zk[k];
+      sumZj = 0.;
+      for (j = 0; j < n; j++)
+        if (j != k) sumZj += zk[j];
+      pI += sumZj * (p[k] - 0.);
+    }
+    pI /= sumZk * (double)(n - 1);
+    for (k = 0; k < n; k++) pI = orc_eos_clamp_p(&eos[k], pI);
+    for (k = 0; k < n; k++) {
+      rhokS[k] = orc_eos_rho_pfinal(&eos[k], eps, pk[k], rhok[k], pI, &drhodp);
+      akS[k] = ak[k] * rhok[k] / rhokS[k];
+      a[k] = akS[k];
+      r[k] = rhokS[k];
+      p[k] = pI;
+    }
+    if (pout) *pout = pI;
+  }
+  return it;
+}
+
+int orc_relax_cell(int n, const orc_eos* eos, double eps, int alpha_null, double* a, double* r, double* p,
+                   double* pstar_out)
+{
+  double cs[MAXPH];
+  int k;
+  (void)alpha_null;
+  for (k = 0; k < n; k++) cs[k] = orc_eos_sound(&eos[k], eps, r[k], p[k]);
+  return relax_state(n, eos, eps, a, r, p, cs, pstar_out);
+}
+
+static void relax_cell(const orc_ctx* c, cst* s, orc_stats* st)
+{
+  int it = relax_state(c->n, c->cfg.eos, c->eps, s->a, s->r, s->p, s->c, NULL);
+  if (st && it > 0) {
+    st->relax_calls++;
+    st->newton_iters += it;
+    if (it >= 100) st->newton_fail++;
+  }
+}
+
+/* Cell::correctionEnergy (Cell.cpp:327-331) -> MixUEq::totalEnergyToInternalEnergy (MixUEq.cpp:175-181)
+   + FluxPUEq::correctionEnergy (FluxPUEq.cpp:68-116); then Cell::fulfillState. */
+static void correct_and_fulfill(const orc_ctx* c, cst* s)
+{
+  int k;
+  if (c->cfg.model == ORC_PUEQ) {
+    double em = s->E - 0.5 * sqn(s->u);
+    double rhoe = s->rho * em;
+    double pp = rhoe, denom = 0., gpg, er, inv, cov;
+    for (k = 0; k < c->n; k++) {
+      mix_eos(EOSK(c, k), &gpg, &er, &inv, &cov);
+      pp -= s->a[k] * (gpg * (1. - s->r[k] * cov) + s->r[k] * er);
+      denom += s->a[k] * inv * (1. - s->r[k] * cov);
+    }
+    pp /= denom;
+    for (k = 0; k < c->n; k++) {
+      s->p[k] = pp;
+      verify_correct(c, s, k);
+    }
+  }
+  fulfill(c, s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mesh bookkeeping (MeshCartesian.cpp:141-147, 168-208, 212-558)            */
+/* ------------------------------------------------------------------------ */
+
+static long long cidx(const orc_ctx* c, int i, int j, int k)
+{
+  return (long long)i + (long long)j * c->cfg.nx + (long long)k * c->cfg.nx * c->cfg.ny;
+}
+
+orc_ctx* orc_create(const orc_config* cfg)
+{
+  orc_ctx* c;
+  int a, n3[3];
+  double L3[3], volume;
+  if (!cfg || cfg->nx < 2 || cfg->ny < 1 || cfg->nz < 1 || cfg->nph < 1 || cfg->nph > MAXPH) return NULL;
+  if (cfg->nz > 1 && cfg->ny <= 1) return NULL;
+  if (cfg->model == ORC_EULER && cfg->nph != 1) return NULL;
+  if (cfg->model != ORC_EULER && cfg->nph < 2) return NULL;
+  c = (orc_ctx*)calloc(1, sizeof(*c));
+  c->cfg = *cfg;
+  if (cfg->model == ORC_PUEQ) c->cfg.relax_p = 1;
+  if (cfg->model == ORC_EULER) c->cfg.relax_p = 0;
+  c->n = cfg->nph;
+  c->D = (cfg->nx > 1) + (cfg->ny > 1) + (cfg->nz > 1);
+  c->ncell = (long long)cfg->nx * cfg->ny * cfg->nz;
+  c->nvar = (cfg->model == ORC_EULER) ? (2 + c->D + 1) : (3 * c->n + c->D + 1);
+  c->eps = (cfg->model != ORC_EULER && cfg->alpha_null) ? 1.e-15 : 0.;
+  n3[0] = cfg->nx; n3[1] = cfg->ny; n3[2] = cfg->nz;
+  L3[0] = cfg->lx; L3[1] = cfg->ly; L3[2] = cfg->lz;
+  for (a = 0; a < 3; a++) {
+    int i;
+    c->d[a] = L3[a] / (double)n3[a];
+    c->pos[a] = (double*)malloc(sizeof(double) * n3[a]);
+    for (i = 0; i < n3[a]; i++) c->pos[a][i] = (i + 0.5) * c->d[a];
+  }
+  volume = c->d[0] * c->d[1] * c->d[2];
+  c->coef[0] = c->d[1] * c->d[2] / volume;
+  c->coef[1] = c->d[0] * c->d[2] / volume;
+  c->coef[2] = c->d[0] * c->d[1] / volume;
+  c->lcfl = 1.e10;
+  if (cfg->nx != 1) c->lcfl = omin(c->lcfl, c->d[0]);
+  if (cfg->ny != 1) c->lcfl = omin(c->lcfl, c->d[1]);
+  if (cfg->nz != 1) c->lcfl = omin(c->lcfl, c->d[2]);
+  if (c->D > 1) c->lcfl *= 0.6;
+  c->W = (cst*)calloc((size_t)c->ncell, sizeof(cst));
+  c->Ws = (cst*)calloc((size_t)c->ncell, sizeof(cst));
+  c->acc = (cons_t*)calloc((size_t)c->ncell, sizeof(cons_t));
+  c->nface_int[0] = (long long)(cfg->nx - 1) * cfg->ny * cfg->nz;
+  c->nface_int[1] = (cfg->ny > 1) ? (long long)cfg->nx * (cfg->ny - 1) * cfg->nz : 0;
+  c->nface_int[2] = (cfg->nz > 1) ? (long long)cfg->nx * cfg->ny * (cfg->nz - 1) : 0;
+  c->face_off[0] = 0;
+  c->face_off[1] = c->nface_int[0];
+  c->face_off[2] = c->nface_int[0] + c->nface_int[1];
+  c->slopes = (slope_t*)calloc((size_t)(c->nface_int[0] + c->nface_int[1] + c->nface_int[2] + 1), sizeof(slope_t));
+  return c;
+}
+
+void orc_destroy(orc_ctx* c)
+{
+  int a;
+  if (!c) return;
+  for (a = 0; a < 3; a++) free(c->pos[a]);
+  free(c->W); free(c->Ws); free(c->acc); free(c->slopes);
+  free(c);
+}
+
+int orc_nvar(const orc_ctx* c) { return c->nvar; }
+long long orc_ncell(const orc_ctx* c) { return c->ncell; }
+
+/* GDHalfSpace.cpp:77-98, GDDisc.cpp:78-96, GDSphere.cpp:60-68, GDRectangle / GDCuboid belong() */
+static int belongs(const orc_domain* d, const double* x)
+{
+  switch (d->kind) {
+    case ORC_DOM_ENTIRE: return 1;
+    case ORC_DOM_HALFSPACE:
+      if (d->dir >= 0) return x[d->axis] >= d->origin;
+      return x[d->axis] <= d->origin;
+    case ORC_DOM_DISC: {
+      double sum = 0.;
+      sum += pow(x[d->axis] - d->center[d->axis], 2.);
+      sum += pow(x[d->axis2] - d->center[d->axis2], 2.);
+      return sum <= d->radius * d->radius;
+    }
+    case ORC_DOM_SPHERE: {
+      double sum = pow(x[0] - d->center[0], 2.) + pow(x[1] - d->center[1], 2.) + pow(x[2] - d->center[2], 2.);
+      return sum <= d->radius * d->radius;
+    }
+    case ORC_DOM_RECTANGLE: {
+      int ax[2], i;
+      ax[0] = d->axis; ax[1] = d->axis2;
+      for (i = 0; i < 2; i++) {
+        if (x[ax[i]] - d->lo[ax[i]] < 0) return 0;
+        if (x[ax[i]] - d->lo[ax[i]] > d->len[ax[i]]) return 0;
+      }
+      return 1;
+    }
+    case ORC_DOM_CUBOID: {
+      int i;
+      for (i = 0; i < 3; i++) {
+        if (x[i] - d->lo[i] < 0) return 0;
+        if (x[i] - d->lo[i] > d->len[i]) return 0;
+      }
+      return 1;
+    }
+  }
+  return 0;
+}
+
+/* Cell::fill (Cell.cpp:133-153) + Cell::completeFulfillState (Cell.cpp:344-353) */
+int orc_fill(orc_ctx* c, const orc_domain* doms, int ndom)
+{
+  int i, j, k, q, m;
+  for (k = 0; k < c->cfg.nz; k++)
+    for (j = 0; j < c->cfg.ny; j++)
+      for (i = 0; i < c->cfg.nx; i++) {
+        double x[3];
+        cst* s = &c->W[cidx(c, i, j, k)];
+        x[0] = c->pos[0][i]; x[1] = c->pos[1][j]; x[2] = c->pos[2][k];
+        memset(s, 0, sizeof(*s));
+        for (q = 0; q < ndom; q++) {
+          if (!belongs(&doms[q], x)) continue;
+          for (m = 0; m < c->n; m++) {
+            s->a[m] = (c->cfg.model == ORC_EULER) ? 1. : doms[q].alpha[m];
+            s->r[m] = doms[q].rho[m];
+            s->p[m] = doms[q].p[m];
+          }
+          s->u[0] = doms[q].vel[0]; s->u[1] = doms[q].vel[1]; s->u[2] = doms[q].vel[2];
+        }
+        fulfill(c, s);
+        if (c->cfg.model != ORC_EULER) s->E = s->em + 0.5 * sqn(s->u);
+      }
+  return 0;
+}
+
+int orc_get_state(const orc_ctx* c, double* out)
+{
+  long long i, N = c->ncell;
+  int v, k, a;
+  for (i = 0; i < N; i++) {
+    const cst* s = &c->W[i];
+    v = 0;
+    if (c->cfg.model == ORC_EULER) {
+      out[(v++) * N + i] = s->r[0];
+      out[(v++) * N + i] = s->p[0];
+    } else {
+      for (k = 0; k < c->n; k++) {
+        out[(v++) * N + i] = s->a[k];
+        out[(v++) * N + i] = s->r[k];
+        out[(v++) * N + i] = s->p[k];
+      }
+    }
+    for (a = 0; a < c->D; a++) out[(v++) * N + i] = s->u[a];
+    out[(v++) * N + i] = s->E;
+  }
+  return 0;
+}
+
+int orc_set_state(orc_ctx* c, const double* in)
+{
+  long long i, N = c->ncell;
+  int v, k, a;
+  for (i = 0; i < N; i++) {
+    cst* s = &c->W[i];
+    memset(s, 0, sizeof(*s));
+    v = 0;
+    if (c->cfg.model == ORC_EULER) {
+      s->a[0] = 1.;
+      s->r[0] = in[(v++) * N + i];
+      s->p[0] = in[(v++) * N + i];
+    } else {
+      for (k = 0; k < c->n; k++) {
+        s->a[k] = in[(v++) * N + i];
+        s->r[k] = in[(v++) * N + i];
+        s->p[k] = in[(v++) * N + i];
+      }
+    }
+    for (a = 0; a < c->D; a++) s->u[a] = in[(v++) * N + i];
+    {
+      double E = in[(v++) * N + i];
+      fulfill(c, s);
+      s->E = E;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Face sweeps                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Boundary face: BoundCond::computeFlux / solveRiemann (BoundCond.cpp:61-98) */
+static void boundary_face(orc_ctx* c, long long L, int fr, int bc, const cst* src, double coef, double* dtMax)
+{
+  cst b = src[L];
+  flux_t F;
+  fulfill(c, &b);
+  local_proj(b.u, fr);
+  if (bc == ORC_BC_WALL) {
+    if (c->cfg.model == ORC_EULER) wall_euler(&b, c->lcfl, dtMax, &F);
+    else wall_ueq(&b, c->lcfl, dtMax, &F);
+  } else {
+    riemann_intern(c, &b, &b, dtMax, &F);
+  }
+  reverse_proj(F.f.mom, fr);
+  sub_flux(c, &c->acc[L], &F, coef, &c->W[L]);
+}
+
+/* First-order interior face: CellInterface::computeFlux / solveRiemann (CellInterface.cpp:114-174) */
+static void interior_face_o1(orc_ctx* c, long long L, long long R, int fr, double coef, double* dtMax)
+{
+  cst l = c->W[L], r = c->W[R];
+  flux_t F;
+  local_proj(l.u, fr);
+  local_proj(r.u, fr);
+  riemann_intern(c, &l, &r, dtMax, &F);
+  reverse_proj(F.f.mom, fr);
+  add_flux(c, &c->acc[R], &F, coef, &c->W[R]);
+  sub_flux(c, &c->acc[L], &F, coef, &c->W[L]);
+}
+
+static void boundary_sweep(orc_ctx* c, const cst* src, double* dtMax)
+{
+  int i, j, k;
+  const orc_config* g = &c->cfg;
+  if (g->nx != 1) {
+    for (j = 0; j < g->ny; j++)
+      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, 0, j, k), F_XM, g->bc[0], src, c->coef[0], dtMax);
+    for (j = 0; j < g->ny; j++)
+      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, g->nx - 1, j, k), F_XP, g->bc[1], src, c->coef[0], dtMax);
+  }
+  if (g->ny != 1) {
+    for (i = 0; i < g->nx; i++)
+      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, i, 0, k), F_YM, g->bc[2], src, c->coef[1], dtMax);
+    for (i = 0; i < g->nx; i++)
+      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, i, g->ny - 1, k), F_YP, g->bc[3], src, c->coef[1], dtMax);
+  }
+  if (g->nz != 1) {
+    for (i = 0; i < g->nx; i++)
+      for (j = 0; j < g->ny; j++) boundary_face(c, cidx(c, i, j, 0), F_ZM, g->bc[4], src, c->coef[2], dtMax);
+    for (i = 0; i < g->nx; i++)
+      for (j = 0; j < g->ny; j++) boundary_face(c, cidx(c, i, j, g->nz - 1), F_ZP, g->bc[5], src, c->coef[2], dtMax);
+  }
+}
+
+static void sweep_o1(orc_ctx* c, double* dtMax)
+{
+  int i, j, k;
+  const orc_config* g = &c->cfg;
+  for (i = 0; i < g->nx - 1; i++)
+    for (j = 0; j < g->ny; j++)
+      for (k = 0; k < g->nz; k++) interior_face_o1(c, cidx(c, i, j, k), cidx(c, i + 1, j, k), F_XI, c->coef[0], dtMax);
+  for (i = 0; i < g->nx; i++)
+    for (j = 0; j < g->ny - 1; j++)
+      for (k = 0; k < g->nz; k++) interior_face_o1(c, cidx(c, i, j, k), cidx(c, i, j + 1, k), F_YI, c->coef[1], dtMax);
+  for (i = 0; i < g->nx; i++)
+    for (j = 0; j < g->ny; j++)
+      for (k = 0; k < g->nz - 1; k++) interior_face_o1(c, cidx(c, i, j, k), cidx(c, i, j, k + 1), F_ZI, c->coef[2], dtMax);
+  boundary_sweep(c, c->W, dtMax);
+}
+
+/* ---- second order ---- */
+
+static long long face_id(const orc_ctx* c, int axis, int i, int j, int k)
+{
+  /* index within the axis list, creation order (ix outer, iy, iz inner) */
+  const orc_config* g = &c->cfg;
+  if (axis == 0) return c->face_off[0] + ((long long)i * g->ny + j) * g->nz + k;
+  if (axis == 1) return c->face_off[1] + ((long long)i * (g->ny - 1) + j) * g->nz + k;
+  return c->face_off[2] + ((long long)i * g->ny + j) * (g->nz - 1) + k;
+}
+
+/* Element::distance (Element.cpp:77-81) and Face::distance (Face.cpp:52-56): sqrt(x*x + y*y + z*z) */
+static double norm3(double x, double y, double z) { return sqrt(x * x + y * y + z * z); }
+
+/* CellInterfaceO2Cartesian::computeSlopes (CellInterfaceO2Cartesian.cpp:77-99) */
+static void compute_slopes(orc_ctx* c, const cst* S)
+{
+  const orc_config* g = &c->cfg;
+  int axis, i, j, k, m;
+  for (axis = 0; axis < 3; axis++) {
+    int n3[3];
+    n3[0] = g->nx; n3[1] = g->ny; n3[2] = g->nz;
+    if (n3[axis] == 1) continue;
+    n3[axis] -= 1;
+    for (i = 0; i < n3[0]; i++)
+      for (j = 0; j < n3[1]; j++)
+        for (k = 0; k < n3[2]; k++) {
+          int iR = i + (axis == 0), jR = j + (axis == 1), kR = k + (axis == 2);
+          long long L = cidx(c, i, j, k), R = cidx(c, iR, jR, kR);
+          slope_t* s = &c->slopes[face_id(c, axis, i, j, k)];
+          double dist = norm3(c->pos[0][i] - c->pos[0][iR], c->pos[1][j] - c->pos[1][jR], c->pos[2][k] - c->pos[2][kR]);
+          for (m = 0; m < c->n; m++) {
+            if (c->cfg.model != ORC_EULER) s->a[m] = (S[R].a[m] - S[L].a[m]) / dist;
+            s->r[m] = (S[R].r[m] - S[L].r[m]) / dist;
+            s->p[m] = (S[R].p[m] - S[L].p[m]) / dist;
+          }
+          s->u[0] = (S[R].u[0] - S[L].u[0]) / dist;
+          s->u[1] = (S[R].u[1] - S[L].u[1]) / dist;
+          s->u[2] = (S[R].u[2] - S[L].u[2]) / dist;
+        }
+  }
+}
+
+static double alpha0(const orc_ctx* c, const cst* s) { return (c->cfg.model == ORC_EULER) ? 0. : s->a[0]; }
+
+/* CellO2Cartesian::computeLocalSlopes (CellO2Cartesian.cpp:23-171) for a uniform Cartesian cell:
+   slope1 = the reference face, slope2 = the opposite face of the same axis if it carries slopes. */
+static void local_slopes(const orc_ctx* c, const slope_t* s1, const slope_t* s2, double aAfter, double aCell,
+                         double aOther, slope_t* out)
+{
+  int k, n = c->n;
+  const int* lim = c->cfg.lim;
+  int lg, lvf;
+  slope_t zero;
+  if (!s2) {
+    memset(&zero, 0, sizeof(zero));
+    s2 = &zero;
+  }
+  if ((aCell >= 1.e-4) && (aCell <= 1. - 1.e-4) && ((aOther - aCell) * (aCell - aAfter) >= 1.e-8)) {
+    lg = lim[1]; lvf = lim[3];
+  } else {
+    lg = lim[0]; lvf = lim[2];
+  }
+  for (k = 0; k < n; k++) {
+    if (c->cfg.model == ORC_EULER) {
+      out->r[k] = orc_limiter(lg, s1->r[k], s2->r[k]);
+      out->p[k] = orc_limiter(lg, s1->p[k], s2->p[k]);
+    } else {
+      out->a[k] = orc_limiter(lvf, s1->a[k], s2->a[k]);
+      out->r[k] = orc_limiter(lg, s1->r[k], s2->r[k]);
+      out->p[k] = orc_limiter(lg, s1->p[k], s2->p[k]);
+    }
+  }
+  out->u[0] = orc_limiter(lg, s1->u[0], s2->u[0]);
+  out->u[1] = orc_limiter(lg, s1->u[1], s2->u[1]);
+  out->u[2] = orc_limiter(lg, s1->u[2], s2->u[2]);
+}
+
+/* PhaseUEq::extrapolate / MixUEq::extrapolate / PhaseEuler::extrapolate + verifyAndCorrectPhase */
+static void extrapolate(const orc_ctx* c, cst* b, const slope_t* s, double dist)
+{
+  int k;
+  for (k = 0; k < c->n; k++) {
+    if (c->cfg.model != ORC_EULER) b->a[k] += s->a[k] * dist;
+    b->r[k] += s->r[k] * dist;
+    b->p[k] += s->p[k] * dist;
+    if (c->cfg.model == ORC_EULER) {
+      b->u[0] = b->u[0] + s->u[0] * dist;
+      b->u[1] = b->u[1] + s->u[1] * dist;
+      b->u[2] = b->u[2] + s->u[2] * dist;
+    }
+    verify_correct(c, b, k);
+  }
+  if (c->cfg.model != ORC_EULER) {
+    b->u[0] = b->u[0] + s->u[0] * dist;
+    b->u[1] = b->u[1] + s->u[1] * dist;
+    b->u[2] = b->u[2] + s->u[2] * dist;
+  }
+}
+
+static void negate_slope(const orc_ctx* c, slope_t* s)
+{
+  int k;
+  for (k = 0; k < c->n; k++) {
+    s->a[k] = -s->a[k];
+    s->r[k] = -s->r[k];
+    s->p[k] = -s->p[k];
+  }
+  s->u[0] = s->u[0] * -1.;
+  s->u[1] = s->u[1] * -1.;
+  s->u[2] = s->u[2] * -1.;
+}
+
+/* CellInterfaceO2Cartesian::solveRiemann (CellInterfaceO2Cartesian.cpp:101-202) + CellInterfaceO2::computeFlux */
+static void interior_face_o2(orc_ctx* c, int axis, int i, int j, int k, const cst* S, double* dtMax)
+{
+  const orc_config* g = &c->cfg;
+  int n3[3], iL[3], iR[3];
+  long long L, R;
+  cst bL, bR;
+  slope_t lim;
+  const slope_t *sref, *sopp;
+  double dL, dR, aL, aR, aLL = 0., aRR = 0., fpos[3];
+  flux_t F;
+  int fr = (axis == 0) ? F_XI : (axis == 1) ? F_YI : F_ZI;
+  n3[0] = g->nx; n3[1] = g->ny; n3[2] = g->nz;
+  iL[0] = i; iL[1] = j; iL[2] = k;
+  iR[0] = i; iR[1] = j; iR[2] = k;
+  iR[axis] += 1;
+  L = cidx(c, iL[0], iL[1], iL[2]);
+  R = cidx(c, iR[0], iR[1], iR[2]);
+  bL = S[L];
+  bR = S[R];
+  fpos[0] = c->pos[0][i]; fpos[1] = c->pos[1][j]; fpos[2] = c->pos[2][k];
+  fpos[axis] = c->pos[axis][iL[axis]] + 0.5 * c->d[axis];
+  dL = norm3(fpos[0] - c->pos[0][iL[0]], fpos[1] - c->pos[1][iL[1]], fpos[2] - c->pos[2][iL[2]]);
+  dR = norm3(fpos[0] - c->pos[0][iR[0]], fpos[1] - c->pos[1][iR[1]], fpos[2] - c->pos[2][iR[2]]);
+  aL = alpha0(c, &bL);
+  aR = alpha0(c, &bR);
+  sref = &c->slopes[face_id(c, axis, i, j, k)];
+  /* left cell: opposite face is (L-1 | L) */
+  sopp = NULL;
+  if (iL[axis] >= 1) {
+    int t[3];
+    t[0] = iL[0]; t[1] = iL[1]; t[2] = iL[2];
+    t[axis] -= 1;
+    sopp = &c->slopes[face_id(c, axis, t[0], t[1], t[2])];
+    aLL = alpha0(c, &c->W[cidx(c, t[0], t[1], t[2])]);
+  }
+  local_slopes(c, sref, sopp, aLL, aL, aR, &lim);
+  extrapolate(c, &bL, &lim, dL);
+  /* right cell: opposite face is (R | R+1) */
+  sopp = NULL;
+  if (iR[axis] <= n3[axis] - 2) {
+    int t[3];
+    t[0] = iR[0]; t[1] = iR[1]; t[2] = iR[2];
+    sopp = &c->slopes[face_id(c, axis, t[0], t[1], t[2])];
+    t[axis] += 1;
+    aRR = alpha0(c, &c->W[cidx(c, t[0], t[1], t[2])]);
+  }
+  local_slopes(c, sref, sopp, aRR, aR, aL, &lim);
+  negate_slope(c, &lim);
+  extrapolate(c, &bR, &lim, dR);
+  fulfill(c, &bL);
+  fulfill(c, &bR);
+  local_proj(bL.u, fr);
+  local_proj(bR.u, fr);
+  riemann_intern(c, &bL, &bR, dtMax, &F);
+  reverse_proj(F.f.mom, fr);
+  add_flux(c, &c->acc[R], &F, c->coef[axis], &c->W[R]);
+  sub_flux(c, &c->acc[L], &F, c->coef[axis], &c->W[L]);
+}
+
+static void sweep_o2(orc_ctx* c, const cst* S, double* dtMax)
+{
+  const orc_config* g = &c->cfg;
+  int i, j, k;
+  for (i = 0; i < g->nx - 1; i++)
+    for (j = 0; j < g->ny; j++)
+      for (k = 0; k < g->nz; k++) interior_face_o2(c, 0, i, j, k, S, dtMax);
+  for (i = 0; i < g->nx; i++)
+    for (j = 0; j < g->ny - 1; j++)
+      for (k = 0; k < g->nz; k++) interior_face_o2(c, 1, i, j, k, S, dtMax);
+  for (i = 0; i < g->nx; i++)
+    for (j = 0; j < g->ny; j++)
+      for (k = 0; k < g->nz - 1; k++) interior_face_o2(c, 2, i, j, k, S, dtMax);
+  boundary_sweep(c, S, dtMax);
+}
+
+/* Run::solveRelaxations (Run.cpp:771-804) applied to one state */
+static void relax_and_correct(orc_ctx* c, cst* s, orc_stats* st)
+{
+  if (c->cfg.relax_p) {
+    relax_cell(c, s, st);
+    correct_and_fulfill(c, s);
+  }
+}
+
+int orc_step(orc_ctx* c, double dt, double* dtmax_out, orc_stats* st)
+{
+  long long i;
+  double dtMax = 1.e10;
+  memset(c->acc, 0, sizeof(cons_t) * (size_t)c->ncell);
+  if (c->cfg.order == 1) {
+    /* Run::solveHyperbolic (Run.cpp:696-716) */
+    sweep_o1(c, &dtMax);
+    for (i = 0; i < c->ncell; i++) {
+      cst old = c->W[i];
+      time_evolution(c, &c->acc[i], &old, dt);
+      build_prim(c, &c->acc[i], &c->W[i]);
+    }
+  } else {
+    /* Run::solveHyperbolicO2 (Run.cpp:627-692) with slopes from integrationProcedure (Run.cpp:526-533) */
+    compute_slopes(c, c->W);
+    sweep_o2(c, c->W, &dtMax);
+    for (i = 0; i < c->ncell; i++) {
+      /* CellO2::predictionOrdre2 (CellO2.cpp:111-135) */
+      c->Ws[i] = c->W[i];
+      time_evolution(c, &c->acc[i], &c->W[i], 0.5 * dt);
+      build_prim(c, &c->acc[i], &c->Ws[i]);
+      if (c->cfg.relax_p) relax_cell(c, &c->Ws[i], st);
+      correct_and_fulfill(c, &c->Ws[i]);
+    }
+    memset(c->acc, 0, sizeof(cons_t) * (size_t)c->ncell);
+    compute_slopes(c, c->Ws);
+    sweep_o2(c, c->Ws, &dtMax);
+    for (i = 0; i < c->ncell; i++) {
+      cst old = c->W[i];
+      time_evolution(c, &c->acc[i], &old, dt);
+      build_prim(c, &c->acc[i], &c->W[i]);
+    }
+  }
+  /* Run::advancingProcedure step 4 (Run.cpp:593) */
+  for (i = 0; i < c->ncell; i++) relax_and_correct(c, &c->W[i], st);
+  if (dtmax_out) *dtmax_out = dtMax;
+  return 0;
+}
+
+int orc_run(orc_ctx* c, int max_iter, double t_end, double* t_out, double* dt_next_out, int* iter_out,
+            orc_stats* st)
+{
+  /* Run::solver (Run.cpp:385-498), dt0 from Run.cpp:38 */
+  double dt = 1.e-15, t = 0., dtMax;
+  int it = 0, done = 0;
+  while (!done) {
+    orc_step(c, dt, &dtMax, st);
+    t += dt;
+    it++;
+    if (max_iter > 0) {
+      if (it >= max_iter) done = 1;
+    } else {
+      if (t >= t_end) done = 1;
+    }
+    dt = c->cfg.cfl * dtMax;
+  }
+  if (t_out) *t_out = t;
+  if (dt_next_out) *dt_next_out = dt;
+  if (iter_out) *iter_out = it;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Point-wise Riemann entry points for the pinning tests                    */
+/* ------------------------------------------------------------------------ */
+
+void orc_hllc_euler(const double* Lv, const double* Rv, double lcfl, double* out, double* dtmax)
+{
+  cst L, R;
+  flux_t F;
+  double dtm = 1.e10;
+  memset(&L, 0, sizeof(L));
+  memset(&R, 0, sizeof(R));
+  L.r[0] = Lv[0]; L.u[0] = Lv[1]; L.u[1] = Lv[2]; L.u[2] = Lv[3]; L.p[0] = Lv[4]; L.c[0] = Lv[5]; L.E = Lv[6];
+  R.r[0] = Rv[0]; R.u[0] = Rv[1]; R.u[1] = Rv[2]; R.u[2] = Rv[3]; R.p[0] = Rv[4]; R.c[0] = Rv[5]; R.E = Rv[6];
+  riemann_euler(&L, &R, lcfl, lcfl, &dtm, &F);
+  out[0] = F.f.mass[0];
+  out[1] = F.f.mom[0]; out[2] = F.f.mom[1]; out[3] = F.f.mom[2];
+  out[4] = F.f.energM;
+  out[5] = F.sM;
+  if (dtmax) *dtmax = dtm;
+}
+
+void orc_hllc_ueq(int n, const orc_eos* eos, double eps, const double* Lv, const double* Rv, double lcfl,
+                  double* out, double* dtmax)
+{
+  orc_ctx c;
+  cst L, R;
+  flux_t F;
+  int k, o = 0;
+  double dtm = 1.e10;
+  memset(&c, 0, sizeof(c));
+  c.cfg.model = ORC_UEQ;
+  c.n = n;
+  c.eps = eps;
+  for (k = 0; k < n; k++) c.cfg.eos[k] = eos[k];
+  memset(&L, 0, sizeof(L));
+  memset(&R, 0, sizeof(R));
+  for (k = 0; k < n; k++) {
+    L.a[k] = Lv[4 * k]; L.r[k] = Lv[4 * k + 1]; L.p[k] = Lv[4 * k + 2]; L.e[k] = Lv[4 * k + 3];
+    R.a[k] = Rv[4 * k]; R.r[k] = Rv[4 * k + 1]; R.p[k] = Rv[4 * k + 2]; R.e[k] = Rv[4 * k + 3];
+  }
+  L.rho = Lv[4 * n]; L.pm = Lv[4 * n + 1]; L.em = Lv[4 * n + 2]; L.cF = Lv[4 * n + 3];
+  L.u[0] = Lv[4 * n + 4]; L.u[1] = Lv[4 * n + 5]; L.u[2] = Lv[4 * n + 6];
+  R.rho = Rv[4 * n]; R.pm = Rv[4 * n + 1]; R.em = Rv[4 * n + 2]; R.cF = Rv[4 * n + 3];
+  R.u[0] = Rv[4 * n + 4]; R.u[1] = Rv[4 * n + 5]; R.u[2] = Rv[4 * n + 6];
+  riemann_ueq(&c, &L, &R, lcfl, lcfl, &dtm, &F);
+  for (k = 0; k < n; k++) out[o++] = F.f.alpha[k];
+  for (k = 0; k < n; k++) out[o++] = F.f.mass[k];
+  for (k = 0; k < n; k++) out[o++] = F.f.energ[k];
+  out[o++] = F.f.mom[0]; out[o++] = F.f.mom[1]; out[o++] = F.f.mom[2];
+  out[o++] = F.f.energM;
+  out[o++] = F.sM;
+  out[o++] = F.uStar;
+  if (dtmax) *dtmax = dtm;
+}

@@ -1,0 +1,109 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * Plain-C restatement of ECOGEN's explicit finite-volume step on uniform
+ * Cartesian meshes (Euler, Kapila PRESSUREVELOCITYEQ, VELOCITYEQ with infinite
+ * pressure relaxation; first order and MUSCL/midpoint second order), written
+ * from the reference sources' behaviour so that the HIP product path can be
+ * checked against it.  Only the test-suite, the smoke check and the bench's
+ * CPU-baseline leg load this library; the product never links it.
+ *
+ * Structure deliberately follows the reference (face loop in creation order
+ * accumulating into per-cell conservative buffers, then a cell loop), which is
+ * independent from the product's per-cell gather kernels.
+ */
+#ifndef ECOGEN_ORACLE_H
+#define ECOGEN_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAXPH 4
+
+enum { ORC_EULER = 0, ORC_PUEQ = 1, ORC_UEQ = 2 };
+enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
+enum { ORC_LIM_NONE = 0, ORC_LIM_MINMOD = 1, ORC_LIM_VANLEER = 2, ORC_LIM_VANALBADA = 3,
+       ORC_LIM_SUPERBEE = 4, ORC_LIM_MC = 5 };
+enum { ORC_BC_NONREFLECTING = 0, ORC_BC_WALL = 1 };
+enum { ORC_DOM_ENTIRE = 0, ORC_DOM_HALFSPACE = 1, ORC_DOM_DISC = 2, ORC_DOM_SPHERE = 3,
+       ORC_DOM_RECTANGLE = 4, ORC_DOM_CUBOID = 5 };
+
+typedef struct {
+  int type;            /* ORC_EOS_IG / ORC_EOS_SG */
+  double gamma, pinf, cv, eref;
+} orc_eos;
+
+typedef struct {
+  int nx, ny, nz;
+  double lx, ly, lz;
+  int model;           /* ORC_EULER / ORC_PUEQ / ORC_UEQ */
+  int nph;             /* phases (1 for Euler) */
+  orc_eos eos[ORC_MAXPH];
+  int order;           /* 1 or 2 */
+  int lim[4];          /* global, interface, global volume fraction, interface volume fraction */
+  int alpha_null;
+  int relax_p;         /* infinite pressure relaxation (forced on for PUEQ) */
+  int bc[6];           /* X-, X+, Y-, Y+, Z-, Z+ */
+  double cfl;
+} orc_config;
+
+typedef struct {
+  int kind;            /* ORC_DOM_* */
+  int axis, axis2, dir;
+  double origin;
+  double center[3];
+  double radius;
+  double lo[3], len[3];   /* rectangle / cuboid: lower corner and edge lengths */
+  double alpha[ORC_MAXPH], rho[ORC_MAXPH], p[ORC_MAXPH];
+  double vel[3];
+} orc_domain;
+
+typedef struct {
+  long long newton_fail;    /* relaxations that hit 100 iterations */
+  long long newton_iters;   /* total Newton iterations */
+  long long relax_calls;    /* cells relaxed */
+} orc_stats;
+
+typedef struct orc_ctx orc_ctx;
+
+orc_ctx* orc_create(const orc_config* cfg);
+void orc_destroy(orc_ctx* c);
+int orc_nvar(const orc_ctx* c);
+long long orc_ncell(const orc_ctx* c);
+/* Fill the initial condition from domains (later domains override), then complete the state. */
+int orc_fill(orc_ctx* c, const orc_domain* doms, int ndom);
+/* Persistent primitive set, flat SoA [nvar][ncell]. */
+int orc_get_state(const orc_ctx* c, double* out);
+int orc_set_state(orc_ctx* c, const double* in);
+/* One full step of size dt; returns dtMax (min over every Riemann call of the step). */
+int orc_step(orc_ctx* c, double dt, double* dtmax, orc_stats* st);
+/* Reference time loop: dt0 = 1e-15, dt_{n+1} = cfl*dtMax_n; stops after max_iter or when t >= t_end. */
+int orc_run(orc_ctx* c, int max_iter, double t_end, double* t_out, double* dt_next_out, int* iter_out,
+            orc_stats* st);
+
+/* Point-wise kernels exposed for pinning tests. */
+double orc_eos_energy(const orc_eos* q, double eps, double rho, double p);
+double orc_eos_pressure(const orc_eos* q, double rho, double e);
+double orc_eos_sound(const orc_eos* q, double eps, double rho, double p);
+double orc_eos_temperature(const orc_eos* q, double eps, double rho, double p);
+double orc_eos_p_isentropic(const orc_eos* q, double eps, double p0, double rho0, double rhof);
+double orc_eos_rho_pfinal(const orc_eos* q, double eps, double p0, double rho0, double pf, double* drhodp);
+double orc_eos_rho_hugoniot(const orc_eos* q, double eps, double p0, double rho0, double pf);
+double orc_eos_clamp_p(const orc_eos* q, double p);
+double orc_limiter(int kind, double s1, double s2);
+/* Euler HLLC in the face frame: in = (rho,u,v,w,p,c,E) x2, out = mass, mom[3], energy, sM. */
+void orc_hllc_euler(const double* L, const double* R, double lcfl, double* out, double* dtmax);
+/* Multiphase (mixture) HLLC in the face frame for n phases.
+ * L/R layout: [alpha,rho,p,e] x n, then rho_mix, p_mix, e_mix, cF, u, v, w.
+ * out layout: alpha[n], mass[n], energ[n], mom[3], energMix, sM, uStar. */
+void orc_hllc_ueq(int n, const orc_eos* eos, double eps, const double* L, const double* R, double lcfl,
+                  double* out, double* dtmax);
+/* Infinite pressure relaxation of one cell: a/r/p updated in place; returns Newton iterations. */
+int orc_relax_cell(int n, const orc_eos* eos, double eps, int alpha_null, double* a, double* r, double* p,
+                   double* pstar_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,186 @@
+"""ctypes binding of the oracle library (test infrastructure only).
+
+Only the test-suite, ``__graft_entry__.smoke`` and the bench's CPU-baseline leg
+import this module.  It loads ``oracle/libecg_oracle.so`` (built by
+``oracle/Makefile``) and exposes the CPU restatement of ECOGEN's finite-volume
+step on the case description shared with the product binding (see
+``ecogen_amd.case``).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+MAXPH = 4
+MODELS = {"euler": 0, "pueq": 1, "ueq": 2}
+EOS_TYPES = {"IG": 0, "SG": 1}
+LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5}
+BCS = {"nonreflecting": 0, "wall": 1, "symmetry": 1}
+DOMAINS = {"entire": 0, "halfspace": 1, "disc": 2, "sphere": 3, "rectangle": 4, "cuboid": 5}
+
+
+class OrcEos(C.Structure):
+    _fields_ = [("type", C.c_int), ("gamma", C.c_double), ("pinf", C.c_double), ("cv", C.c_double),
+                ("eref", C.c_double)]
+
+
+class OrcConfig(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+                ("model", C.c_int), ("nph", C.c_int), ("eos", OrcEos * MAXPH), ("order", C.c_int),
+                ("lim", C.c_int * 4), ("alpha_null", C.c_int), ("relax_p", C.c_int), ("bc", C.c_int * 6),
+                ("cfl", C.c_double)]
+
+
+class OrcDomain(C.Structure):
+    _fields_ = [("kind", C.c_int), ("axis", C.c_int), ("axis2", C.c_int), ("dir", C.c_int),
+                ("origin", C.c_double), ("center", C.c_double * 3), ("radius", C.c_double),
+                ("lo", C.c_double * 3), ("len", C.c_double * 3),
+                ("alpha", C.c_double * MAXPH), ("rho", C.c_double * MAXPH), ("p", C.c_double * MAXPH),
+                ("vel", C.c_double * 3)]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("newton_fail", C.c_longlong), ("newton_iters", C.c_longlong), ("relax_calls", C.c_longlong)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libecg_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle library missing: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(OrcConfig)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_nvar.argtypes = [C.c_void_p]
+        L.orc_ncell.argtypes = [C.c_void_p]
+        L.orc_ncell.restype = C.c_longlong
+        L.orc_fill.argtypes = [C.c_void_p, C.POINTER(OrcDomain), C.c_int]
+        L.orc_get_state.argtypes = [C.c_void_p, dp]
+        L.orc_set_state.argtypes = [C.c_void_p, dp]
+        L.orc_step.argtypes = [C.c_void_p, C.c_double, dp, C.POINTER(OrcStats)]
+        L.orc_run.argtypes = [C.c_void_p, C.c_int, C.c_double, dp, dp, C.POINTER(C.c_int), C.POINTER(OrcStats)]
+        for name in ("orc_eos_energy", "orc_eos_sound", "orc_eos_temperature"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.POINTER(OrcEos), C.c_double, C.c_double, C.c_double]
+        L.orc_eos_pressure.restype = C.c_double
+        L.orc_eos_pressure.argtypes = [C.POINTER(OrcEos), C.c_double, C.c_double]
+        L.orc_eos_p_isentropic.restype = C.c_double
+        L.orc_eos_p_isentropic.argtypes = [C.POINTER(OrcEos), C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_eos_rho_pfinal.restype = C.c_double
+        L.orc_eos_rho_pfinal.argtypes = [C.POINTER(OrcEos), C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_eos_rho_hugoniot.restype = C.c_double
+        L.orc_eos_rho_hugoniot.argtypes = [C.POINTER(OrcEos), C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_eos_clamp_p.restype = C.c_double
+        L.orc_eos_clamp_p.argtypes = [C.POINTER(OrcEos), C.c_double]
+        L.orc_limiter.restype = C.c_double
+        L.orc_limiter.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.orc_hllc_euler.argtypes = [dp, dp, C.c_double, dp, dp]
+        L.orc_hllc_ueq.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, dp, dp, C.c_double, dp, dp]
+        L.orc_relax_cell.restype = C.c_int
+        L.orc_relax_cell.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, C.c_int, dp, dp, dp, dp]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_eos(e):
+    return OrcEos(EOS_TYPES[e["type"].upper()], float(e["gamma"]), float(e.get("pinf", 0.0)),
+                  float(e.get("cv", 0.0)), float(e.get("eref", 0.0)))
+
+
+def make_config(case):
+    c = OrcConfig()
+    c.nx, c.ny, c.nz = case["nx"], case["ny"], case["nz"]
+    c.lx, c.ly, c.lz = case["lx"], case["ly"], case["lz"]
+    c.model = MODELS[case["model"]]
+    c.nph = len(case["eos"])
+    for k, e in enumerate(case["eos"]):
+        c.eos[k] = make_eos(e)
+    c.order = case.get("order", 1)
+    lims = case.get("limiters", ("none",) * 4)
+    for i in range(4):
+        c.lim[i] = LIMITERS[lims[i].lower()]
+    c.alpha_null = int(bool(case.get("alpha_null", False)))
+    c.relax_p = int(bool(case.get("relax_p", case["model"] == "pueq")))
+    for i, b in enumerate(case.get("bc", ("nonreflecting",) * 6)):
+        c.bc[i] = BCS[b.lower()]
+    c.cfl = case.get("cfl", 0.8)
+    return c
+
+
+def make_domains(case):
+    doms = case["domains"]
+    arr = (OrcDomain * len(doms))()
+    for i, d in enumerate(doms):
+        o = arr[i]
+        o.kind = DOMAINS[d["kind"]]
+        o.axis = d.get("axis", 0)
+        o.axis2 = d.get("axis2", 1)
+        o.dir = d.get("dir", 1)
+        o.origin = d.get("origin", 0.0)
+        for a in range(3):
+            o.center[a] = d.get("center", (0.0, 0.0, 0.0))[a]
+            o.lo[a] = d.get("lo", (0.0, 0.0, 0.0))[a]
+            o.len[a] = d.get("len", (0.0, 0.0, 0.0))[a]
+            o.vel[a] = d.get("vel", (0.0, 0.0, 0.0))[a]
+        o.radius = d.get("radius", 0.0)
+        for k in range(len(d["rho"])):
+            o.alpha[k] = d.get("alpha", [1.0] * len(d["rho"]))[k]
+            o.rho[k] = d["rho"][k]
+            o.p[k] = d["p"][k]
+    return arr
+
+
+class Oracle:
+    """CPU restatement of one ECOGEN case (persistent primitive set in SoA)."""
+
+    def __init__(self, case):
+        self.L = lib()
+        self.cfg = make_config(case)
+        self.h = self.L.orc_create(C.byref(self.cfg))
+        if not self.h:
+            raise ValueError("oracle rejected the case configuration")
+        self.nvar = self.L.orc_nvar(self.h)
+        self.ncell = self.L.orc_ncell(self.h)
+        self.stats = OrcStats()
+        if "domains" in case:
+            doms = make_domains(case)
+            self.L.orc_fill(self.h, doms, len(doms))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.L.orc_destroy(h)
+            self.h = None
+
+    def get_state(self):
+        out = np.empty((self.nvar, self.ncell), dtype=np.float64)
+        self.L.orc_get_state(self.h, _dp(out))
+        return out
+
+    def set_state(self, st):
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        assert st.shape == (self.nvar, self.ncell)
+        self.L.orc_set_state(self.h, _dp(st))
+
+    def step(self, dt):
+        dtm = np.zeros(1)
+        self.L.orc_step(self.h, float(dt), _dp(dtm), C.byref(self.stats))
+        return float(dtm[0])
+
+    def run(self, max_iter=0, t_end=0.0):
+        t = np.zeros(1)
+        dtn = np.zeros(1)
+        it = C.c_int(0)
+        self.L.orc_run(self.h, int(max_iter), float(t_end), _dp(t), _dp(dtn), C.byref(it), C.byref(self.stats))
+        return float(t[0]), float(dtn[0]), int(it.value)

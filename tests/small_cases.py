@@ -1,0 +1,113 @@
+"""Small replicas of the benchmark configurations used by the parity tests.
+
+Sizes are chosen so that the oracle (plain C, one core) finishes each case in
+well under a second per step, while every code path of the step is exercised:
+interior and boundary faces on every axis, wall and transmissive boundaries,
+first and second order, 2 and 3 phases, with and without alphaNull.
+"""
+from ecogen_amd.case import IG_AIR, SG_EPOXY, SG_WATER, kapila_bubbles_2d, shocked_state, sod_case, \
+    ueq2_pockets_3d, ueq3_spheres_3d
+
+
+def euler_sod(order=1, n=200):
+    c = sod_case(n=n)
+    c["order"] = order
+    if order == 2:
+        c["limiters"] = ("minmod",) * 4
+    c["name"] = f"euler_1d_o{order}"
+    return c
+
+
+def euler_2d(order=1, nx=40, ny=32, bc=("nonreflecting",) * 6, lim="minmod"):
+    doms = [dict(kind="entire", rho=[1.0], p=[1.e5], vel=[0., 0., 0.]),
+            dict(kind="halfspace", axis=0, origin=0.3, dir=-1, rho=[3.0], p=[4.e5], vel=[80., 0., 0.]),
+            dict(kind="disc", axis=0, axis2=1, center=(0.6, 0.45, 0.), radius=0.2, rho=[0.3], p=[1.e5],
+                 vel=[0., -30., 0.])]
+    return dict(name=f"euler_2d_o{order}" + ("" if lim == "minmod" else "_" + lim), nx=nx, ny=ny, nz=1, lx=1.0, ly=0.8, lz=0.05, model="euler",
+                eos=[dict(IG_AIR)], order=order, limiters=(lim,) * 4 if order == 2 else ("none",) * 4,
+                cfl=0.8, bc=bc, domains=doms)
+
+
+def euler_3d(order=1, n=12, bc=("nonreflecting",) * 6):
+    doms = [dict(kind="entire", rho=[1.0], p=[1.e5], vel=[0., 0., 0.]),
+            dict(kind="sphere", center=(0.5, 0.45, 0.55), radius=0.3, rho=[4.0], p=[6.e5], vel=[10., -20., 30.])]
+    return dict(name=f"euler_3d_o{order}", nx=n, ny=n + 2, nz=n + 1, lx=1.0, ly=1.1, lz=1.05, model="euler",
+                eos=[dict(IG_AIR)], order=order, limiters=("minmod",) * 4 if order == 2 else ("none",) * 4,
+                cfl=0.8, bc=bc, domains=doms)
+
+
+def pueq_2d(order=1, n=40, alpha_null=False, bc=("nonreflecting",) * 6):
+    c = kapila_bubbles_2d(n=n, nbub=6, order=order)
+    c["ny"] = n - 8
+    c["ly"] = (n - 8) / n
+    c["alpha_null"] = alpha_null
+    c["bc"] = bc
+    c["name"] = f"pueq_2d_o{order}" + ("_an" if alpha_null else "")
+    return c
+
+
+def pueq_1d(order=1, n=120):
+    p0, p1 = 1.e5, 1.e9
+    rho1, u1 = shocked_state(SG_WATER, p0, 1000.0, p1)
+    e = 1.e-6
+    doms = [dict(kind="entire", alpha=[1 - e, e], rho=[1000., 1.2], p=[p0, p0], vel=[0., 0., 0.]),
+            dict(kind="halfspace", axis=0, origin=0.6, dir=1, alpha=[e, 1 - e], rho=[1000., 1.2], p=[p0, p0],
+                 vel=[0., 0., 0.]),
+            dict(kind="halfspace", axis=0, origin=0.2, dir=-1, alpha=[1 - e, e], rho=[rho1, 1.2], p=[p1, p1],
+                 vel=[u1, 0., 0.])]
+    return dict(name=f"pueq_1d_o{order}", nx=n, ny=1, nz=1, lx=1.0, ly=0.1, lz=0.1, model="pueq",
+                eos=[dict(SG_WATER), dict(IG_AIR)], order=order,
+                limiters=("minmod",) * 4 if order == 2 else ("none",) * 4, cfl=0.8, alpha_null=False,
+                bc=("nonreflecting",) * 6, domains=doms)
+
+
+def pueq3_3d(order=1, n=10):
+    e = 1.e-6
+    doms = [dict(kind="entire", alpha=[1 - 2 * e, e, e], rho=[1000., 1185., 1.2], p=[1.e5] * 3, vel=[0., 0., 0.]),
+            dict(kind="sphere", center=(0.5, 0.5, 0.5), radius=0.25, alpha=[e, 1 - 2 * e, e], rho=[1000., 1185., 1.2],
+                 p=[1.e5] * 3, vel=[0., 0., 0.]),
+            dict(kind="sphere", center=(0.7, 0.3, 0.4), radius=0.2, alpha=[e, e, 1 - 2 * e], rho=[1000., 1185., 1.2],
+                 p=[1.e5] * 3, vel=[0., 0., 0.]),
+            dict(kind="halfspace", axis=0, origin=0.25, dir=-1, alpha=[1 - 2 * e, e, e], rho=[1200., 1185., 1.2],
+                 p=[5.e8] * 3, vel=[150., 0., 0.])]
+    return dict(name=f"pueq3_3d_o{order}", nx=n, ny=n + 1, nz=n + 2, lx=1.0, ly=1.1, lz=1.2, model="pueq",
+                eos=[dict(SG_WATER), dict(SG_EPOXY), dict(IG_AIR)], order=order,
+                limiters=("minmod",) * 4 if order == 2 else ("none",) * 4, cfl=0.8, alpha_null=False,
+                bc=("nonreflecting",) * 6, domains=doms)
+
+
+def ueq_2d(order=1, n=36, alpha_null=False):
+    c = kapila_bubbles_2d(n=n, nbub=5, order=order, model="ueq")
+    c["ny"] = n - 4
+    c["ly"] = (n - 4) / n
+    c["alpha_null"] = alpha_null
+    c["name"] = f"ueq_2d_o{order}" + ("_an" if alpha_null else "")
+    return c
+
+
+def ueq3_3d(order=1, n=12):
+    c = ueq3_spheres_3d(n=n, nsph=6, order=order)
+    c["name"] = f"ueq3_3d_o{order}"
+    return c
+
+
+def ueq2_3d(order=2, n=12):
+    c = ueq2_pockets_3d(n=n, npockets=12, order=order)
+    c["name"] = f"ueq2_3d_o{order}"
+    return c
+
+
+def wall_variant(case, bc=("wall", "nonreflecting", "wall", "wall", "nonreflecting", "wall")):
+    c = dict(case)
+    c["bc"] = bc
+    c["name"] = case["name"] + "_wall"
+    return c
+
+
+def all_cases():
+    cs = [euler_sod(1), euler_sod(2), euler_2d(1), euler_2d(2), euler_2d(2, lim="superbee"), euler_3d(1),
+          euler_3d(2), wall_variant(euler_2d(1)), wall_variant(euler_3d(1)),
+          pueq_1d(1), pueq_1d(2), pueq_2d(1), pueq_2d(2), pueq_2d(1, alpha_null=True), wall_variant(pueq_2d(1)),
+          pueq3_3d(1), pueq3_3d(2),
+          ueq_2d(1), ueq_2d(2), ueq_2d(1, alpha_null=True), ueq3_3d(1), ueq2_3d(2), wall_variant(ueq3_3d(1))]
+    return {c["name"]: c for c in cs}

@@ -1,0 +1,598 @@
+"""Pins of the oracle (CPU restatement of the reference step).
+
+Each check is chosen so that a plausible restatement mistake (dropped term,
+wrong sign or index, transposed operand, wrong association) fails it:
+
+* EOS and limiters: bit-for-bit against the reference's own EosSG / EosIG /
+  Limiter* code (vectors in tests/golden/eos_vectors.npz, written by
+  tests/golden/make_eos_vectors.py from the reference-compiled checker; the
+  live checker is used too when it has been built here),
+* the whole Euler time loop: bit-for-bit against a run of the reference binary
+  (iteration count, final time and next time step of the 1000-cell shock tube),
+* closed forms: the relaxation density solves the energy relation it is derived
+  from, the Hugoniot density satisfies Rankine-Hugoniot, d(rho)/dp matches a
+  finite difference,
+* HLLC: supersonic and uniform states give the physical flux, the star states
+  agree with Toro's independent formulation, mirror symmetry,
+* multiphase HLLC and full steps: contact preservation (uniform p and u across
+  a volume-fraction jump), conservation of mass per phase, momentum and total
+  energy, single-material limit equal to the Euler solver,
+* relaxation: sum of volume fractions 1, equal pressures, alpha_k*rho_k kept,
+  root equal to bisection,
+* energy closures: the VELOCITYEQ scheme correction makes the phase energies
+  add up to the mixture internal energy; the Kapila correction makes the
+  mixture EOS consistent,
+* order of accuracy of the MUSCL/midpoint scheme and convergence to the exact
+  Riemann solution.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from oracle.pyoracle import Oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden", "eos_vectors.npz")
+REF_EOS = os.path.join(ROOT, "oracle", "_ref", "ref_eos")
+
+L = po.lib()
+DP = C.POINTER(C.c_double)
+
+
+def eos_struct(kind, g, pinf, cv, eref):
+    return po.OrcEos(kind, g, pinf, cv, eref)
+
+
+def oracle_eval(op, kind, v):
+    """Evaluate one golden record with the oracle's point-wise functions."""
+    g, pinf, cv, eref, epsflag, a, b, c = v
+    eps = 1.e-15 if epsflag > 0 else 0.0
+    e = eos_struct(kind, g, pinf, cv, eref)
+    d = C.c_double(0.0)
+    if op == 0:
+        return L.orc_eos_energy(C.byref(e), eps, a, b), 0.0
+    if op == 1:
+        return L.orc_eos_pressure(C.byref(e), a, b), 0.0
+    if op == 2:
+        return L.orc_eos_sound(C.byref(e), eps, a, b), 0.0
+    if op == 3:
+        return L.orc_eos_temperature(C.byref(e), eps, a, b), 0.0
+    if op == 4:
+        return L.orc_eos_p_isentropic(C.byref(e), eps, a, b, c), 0.0
+    if op == 5:
+        r = L.orc_eos_rho_pfinal(C.byref(e), eps, a, b, c, C.byref(d))
+        return r, d.value
+    if op == 6:
+        r = L.orc_eos_rho_hugoniot(C.byref(e), eps, a, b, c)
+        return r, None
+    if op == 7:
+        return L.orc_eos_clamp_p(C.byref(e), a), 0.0
+    if op == 8:
+        return L.orc_limiter(kind, a, b), 0.0
+    raise ValueError(op)
+
+
+def _check_records(op, kind, v, out):
+    bad = 0
+    for i in range(len(op)):
+        r0, r1 = oracle_eval(int(op[i]), int(kind[i]), v[i])
+        same0 = (r0 == out[i, 0]) or (np.isnan(r0) and np.isnan(out[i, 0]))
+        same1 = r1 is None or (r1 == out[i, 1]) or (np.isnan(r1) and np.isnan(out[i, 1]))
+        if not (same0 and same1):
+            bad += 1
+    return bad
+
+
+def test_eos_and_limiters_bitwise_vs_reference_vectors():
+    z = np.load(GOLDEN)
+    assert len(z["op"]) > 5000
+    assert _check_records(z["op"], z["kind"], z["v"], z["out"]) == 0
+
+
+@pytest.mark.skipif(not os.path.exists(REF_EOS), reason="reference-compiled checker not built here")
+def test_eos_and_limiters_bitwise_vs_live_reference():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_eos_vectors as mk
+    arr = mk.records(seed=12345)
+    out = np.frombuffer(subprocess.run([REF_EOS], input=arr.tobytes(), capture_output=True, check=True).stdout,
+                        dtype="<f8").reshape(-1, 2)
+    assert _check_records(arr["op"], arr["kind"], arr["v"], out) == 0
+
+
+def test_sod_time_loop_bitwise_vs_reference_run():
+    """1000-cell shock tube (y/z extent 0.1), t_end 5e-4: the reference run ends after
+    433 iterations at t = 5.0106045125640768e-4 with dt_next = 1.1542745158200889e-6."""
+    from ecogen_amd.case import sod_case
+    o = Oracle(sod_case(1000))
+    t, dtn, it = o.run(0, 5.e-4)
+    assert it == 433
+    assert t == float("0.00050106045125640768")
+    assert dtn == float("1.1542745158200889e-06")
+
+
+# --------------------------------------------------------------------------------------------
+# EOS closed forms
+# --------------------------------------------------------------------------------------------
+
+EOSES = [(1, 4.4, 6.e8), (1, 2.43, 5.3e9), (0, 1.4, 0.0)]
+
+
+@pytest.mark.parametrize("kind,g,pinf", EOSES)
+def test_pfinal_solves_its_energy_relation(kind, g, pinf):
+    """rho* = computeDensityPfinal(p0, rho0, p*) solves e(rho*,p*) - e(rho0,p0) + p*(1/rho* - 1/rho0) = 0."""
+    rng = np.random.default_rng(1)
+    e = eos_struct(kind, g, pinf, 1000.0, 0.0)
+    d = C.c_double()
+    for _ in range(200):
+        rho0 = 10 ** rng.uniform(-1, 3)
+        p0 = 10 ** rng.uniform(4, 9)
+        pf = 10 ** rng.uniform(4, 9)
+        rs = L.orc_eos_rho_pfinal(C.byref(e), 0.0, p0, rho0, pf, C.byref(d))
+        res = (L.orc_eos_energy(C.byref(e), 0.0, rs, pf) - L.orc_eos_energy(C.byref(e), 0.0, rho0, p0)
+               + pf * (1.0 / rs - 1.0 / rho0))
+        scale = abs(L.orc_eos_energy(C.byref(e), 0.0, rho0, p0)) + abs(pf / rs)
+        assert abs(res) <= 1e-12 * scale
+        # derivative against a central difference
+        h = 1e-4 * (pf + pinf)
+        fd = (L.orc_eos_rho_pfinal(C.byref(e), 0.0, p0, rho0, pf + h, None)
+              - L.orc_eos_rho_pfinal(C.byref(e), 0.0, p0, rho0, pf - h, None)) / (2 * h)
+        assert abs(d.value - fd) <= 1e-6 * abs(fd)
+
+
+@pytest.mark.parametrize("kind,g,pinf", EOSES)
+def test_hugoniot_and_isentrope(kind, g, pinf):
+    rng = np.random.default_rng(2)
+    e = eos_struct(kind, g, pinf, 1000.0, 0.0)
+    for _ in range(100):
+        rho0 = 10 ** rng.uniform(-1, 3)
+        p0 = 10 ** rng.uniform(4, 8)
+        p1 = p0 * 10 ** rng.uniform(0, 3)
+        r1 = L.orc_eos_rho_hugoniot(C.byref(e), 0.0, p0, rho0, p1)
+        # Rankine-Hugoniot energy: e1 - e0 = (p1 + p0)/2 (v0 - v1)
+        e1 = L.orc_eos_energy(C.byref(e), 0.0, r1, p1)
+        e0 = L.orc_eos_energy(C.byref(e), 0.0, rho0, p0)
+        rhs = 0.5 * (p1 + p0) * (1.0 / rho0 - 1.0 / r1)
+        assert abs((e1 - e0) - rhs) <= 1e-13 * (abs(e1) + abs(e0) + abs(rhs))
+        # isentrope: (p + pinf) / rho^gamma is invariant
+        rf = rho0 * 10 ** rng.uniform(-0.3, 0.3)
+        pf = L.orc_eos_p_isentropic(C.byref(e), 0.0, p0, rho0, rf)
+        assert abs((pf + pinf) / rf ** g - (p0 + pinf) / rho0 ** g) <= 1e-12 * (p0 + pinf) / rho0 ** g
+        # p(rho, e(rho, p)) = p and c^2 = gamma (p + pinf) / rho
+        en = L.orc_eos_energy(C.byref(e), 0.0, rho0, p0)
+        assert abs(L.orc_eos_pressure(C.byref(e), rho0, en) - p0) <= 1e-9 * (p0 + pinf)
+        c = L.orc_eos_sound(C.byref(e), 0.0, rho0, p0)
+        assert abs(c * c - g * (p0 + pinf) / rho0) <= 1e-13 * c * c
+
+
+def test_limiters_closed_forms():
+    rng = np.random.default_rng(3)
+    for _ in range(500):
+        a, b = rng.normal(size=2) * 10 ** rng.uniform(-3, 3, size=2)
+        mm = L.orc_limiter(po.LIMITERS["minmod"], a, b)
+        if a * b > 1e-9:
+            assert mm == np.sign(a) * min(abs(a), abs(b))
+        else:
+            assert mm == 0.0
+        vl = L.orc_limiter(po.LIMITERS["vanleer"], a, b)
+        if a * b > 1e-6 and abs(a + b) > 1e-6:
+            assert abs(vl - 2 * a * b / (a + b)) <= 1e-15 * abs(vl)
+        else:
+            assert vl == 0.0
+        mc = L.orc_limiter(po.LIMITERS["mc"], a, b)
+        if a * b > 1e-9:
+            assert mc == np.sign(a) * min(2 * abs(a), 2 * abs(b), 0.5 * (abs(a) + abs(b)))
+        sb = L.orc_limiter(po.LIMITERS["superbee"], a, b)
+        if a * b > 1e-6:
+            assert sb == np.sign(a) * max(min(2 * abs(a), abs(b)), min(abs(a), 2 * abs(b)))
+    # absolute thresholds of the reference (LimiterMinmod.cpp:48: 1e-9, VanLeer: 1e-6)
+    assert L.orc_limiter(po.LIMITERS["minmod"], 3e-5, 3e-5) == 0.0
+    assert L.orc_limiter(po.LIMITERS["minmod"], 4e-5, 4e-5) == 4e-5
+    assert L.orc_limiter(po.LIMITERS["vanleer"], 1e-3, 1e-3) == 0.0
+
+
+# --------------------------------------------------------------------------------------------
+# Riemann solvers
+# --------------------------------------------------------------------------------------------
+
+GAM = 1.4
+
+
+def _euler_state(rho, u, v, w, p):
+    c = np.sqrt(GAM * p / rho)
+    E = p / (GAM - 1) / rho + 0.5 * (u * u + v * v + w * w)
+    return np.array([rho, u, v, w, p, c, E])
+
+
+def _hllc_euler(Ls, Rs):
+    out = np.zeros(6)
+    dtm = np.zeros(1)
+    L.orc_hllc_euler(Ls.ctypes.data_as(DP), Rs.ctypes.data_as(DP), 1.0, out.ctypes.data_as(DP), dtm.ctypes.data_as(DP))
+    return out, dtm[0]
+
+
+def _phys_flux(s):
+    rho, u, v, w, p, c, E = s
+    return np.array([rho * u, rho * u * u + p, rho * v * u, rho * w * u, (rho * E + p) * u])
+
+
+def _toro_hllc(Ls, Rs):
+    """Toro's HLLC (Davis speeds): F*_K = F_K + S_K (U*_K - U_K)."""
+    rL, uL, vL, wL, pL, cL, EL = Ls
+    rR, uR, vR, wR, pR, cR, ER = Rs
+    sL = min(uL - cL, uR - cR)
+    sR = max(uR + cR, uL + cL)
+    sM = (pR - pL + rL * uL * (sL - uL) - rR * uR * (sR - uR)) / (rL * (sL - uL) - rR * (sR - uR))
+    if sL > 0:
+        return _phys_flux(Ls), sM
+    if sR < 0:
+        return _phys_flux(Rs), sM
+    K, sK = (Ls, sL) if sM >= 0 else (Rs, sR)
+    r, u, v, w, p, c, E = K
+    f = r * (sK - u) / (sK - sM)
+    Us = f * np.array([1.0, sM, v, w, E + (sM - u) * (sM + p / (r * (sK - u)))])
+    U = np.array([r, r * u, r * v, r * w, r * E])
+    return _phys_flux(K) + sK * (Us - U), sM
+
+
+def test_euler_hllc_against_toro_and_symmetries():
+    rng = np.random.default_rng(4)
+    for _ in range(500):
+        Ls = _euler_state(10 ** rng.uniform(-1, 1), rng.normal() * 300, rng.normal() * 50, rng.normal() * 50,
+                          10 ** rng.uniform(4, 6))
+        Rs = _euler_state(10 ** rng.uniform(-1, 1), rng.normal() * 300, rng.normal() * 50, rng.normal() * 50,
+                          10 ** rng.uniform(4, 6))
+        F, dtm = _hllc_euler(Ls, Rs)
+        T, sM = _toro_hllc(Ls, Rs)
+        scale = np.abs(_phys_flux(Ls)) + np.abs(_phys_flux(Rs)) + 1.0
+        assert np.all(np.abs(F[:5] - T) <= 1e-9 * scale)
+        assert abs(F[5] - (0.0 if abs(sM) < 1e-8 else sM)) <= 1e-9 * (abs(sM) + 1)
+        # mirror symmetry x -> -x: swap sides and flip normal velocities
+        mL = Rs.copy(); mL[1] = -Rs[1]
+        mR = Ls.copy(); mR[1] = -Ls[1]
+        Fm, _ = _hllc_euler(mL, mR)
+        assert abs(Fm[0] + F[0]) <= 1e-9 * scale[0]
+        assert abs(Fm[1] - F[1]) <= 1e-9 * scale[1]
+        assert abs(Fm[4] + F[4]) <= 1e-9 * scale[4]
+        # CFL bound: lCFL / max |s|
+        sL = min(Ls[1] - Ls[5], Rs[1] - Rs[5])
+        sR = max(Rs[1] + Rs[5], Ls[1] + Ls[5])
+        assert dtm == min(1.0 / abs(sL), 1.0 / abs(sR))
+    # uniform and supersonic states give the physical flux exactly
+    s = _euler_state(1.2, 30.0, 4.0, -2.0, 1.e5)
+    F, _ = _hllc_euler(s, s)
+    assert np.allclose(F[:5], _phys_flux(s), rtol=1e-14, atol=0)
+    sup = _euler_state(1.0, 2000.0, 1.0, 0.0, 1.e5)
+    sup2 = _euler_state(0.5, 2100.0, 0.0, 0.0, 2.e5)
+    F, _ = _hllc_euler(sup, sup2)
+    assert np.array_equal(F[:5], _phys_flux(sup))
+
+
+SG_W = (1, 4.4, 6.e8, 1000.0, 0.0)
+IG_A = (0, 1.4, 0.0, 800.0, 0.0)
+
+
+def _mix_state(eos, a, r, p, u, v, w):
+    """[alpha, rho, p, e] x n, then rho, p, e, cF, u, v, w (MixUEq::computeMixtureVariables)."""
+    n = len(a)
+    es = [eos_struct(*q) for q in eos]
+    e = [L.orc_eos_energy(C.byref(es[k]), 0.0, r[k], p[k]) for k in range(n)]
+    c = [L.orc_eos_sound(C.byref(es[k]), 0.0, r[k], p[k]) for k in range(n)]
+    rho = sum(a[k] * r[k] for k in range(n))
+    pm = sum(a[k] * p[k] for k in range(n))
+    Y = [a[k] * r[k] / rho for k in range(n)]
+    em = sum(Y[k] * e[k] for k in range(n))
+    cf = np.sqrt(sum(Y[k] * c[k] * c[k] for k in range(n)))
+    out = []
+    for k in range(n):
+        out += [a[k], r[k], p[k], e[k]]
+    return np.array(out + [rho, pm, em, cf, u, v, w])
+
+
+def _hllc_ueq(eos, Ls, Rs):
+    n = len(eos)
+    arr = (po.OrcEos * n)(*[eos_struct(*q) for q in eos])
+    out = np.zeros(3 * n + 6)
+    dtm = np.zeros(1)
+    L.orc_hllc_ueq(n, arr, 0.0, Ls.ctypes.data_as(DP), Rs.ctypes.data_as(DP), 1.0, out.ctypes.data_as(DP),
+                   dtm.ctypes.data_as(DP))
+    return out
+
+
+def test_multiphase_hllc_single_material_limit_equals_euler():
+    """Two phases of the same ideal gas with the same state reduce to the Euler HLLC."""
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        rl, rr = 10 ** rng.uniform(-1, 1, size=2)
+        pl, pr = 10 ** rng.uniform(4, 6, size=2)
+        ul, ur = rng.normal(size=2) * 200
+        vl, vr = rng.normal(size=2) * 20
+        al = rng.uniform(0.1, 0.9)
+        ar = rng.uniform(0.1, 0.9)
+        Ls = _mix_state([IG_A, IG_A], [al, 1 - al], [rl, rl], [pl, pl], ul, vl, 0.0)
+        Rs = _mix_state([IG_A, IG_A], [ar, 1 - ar], [rr, rr], [pr, pr], ur, vr, 0.0)
+        F = _hllc_ueq([IG_A, IG_A], Ls, Rs)
+        E, _ = _hllc_euler(_euler_state(rl, ul, vl, 0.0, pl), _euler_state(rr, ur, vr, 0.0, pr))
+        n = 2
+        mass = F[n] + F[n + 1]
+        scale = abs(E[0]) + rl * abs(ul) + rr * abs(ur) + 1
+        assert abs(mass - E[0]) <= 1e-10 * scale
+        assert abs(F[3 * n] - E[1]) <= 1e-10 * (abs(E[1]) + pl + pr)
+        assert abs(F[3 * n + 1] - E[2]) <= 1e-9 * (abs(E[2]) + 1)
+        # mixture energy flux equals the Euler energy flux
+        assert abs(F[3 * n + 3] - E[4]) <= 1e-9 * (abs(E[4]) + 1)
+
+
+def test_multiphase_hllc_preserves_contact():
+    """Uniform p and u across a volume-fraction / material jump: the star velocity is u,
+    the momentum flux p + rho u^2 of the upwind side, volume-fraction flux alpha_upwind * u."""
+    for u in (0.0, 35.0, -20.0):
+        Ls = _mix_state([SG_W, IG_A], [1 - 1e-6, 1e-6], [1000.0, 1.2], [1e5, 1e5], u, 0.0, 0.0)
+        Rs = _mix_state([SG_W, IG_A], [1e-6, 1 - 1e-6], [1000.0, 1.2], [1e5, 1e5], u, 0.0, 0.0)
+        F = _hllc_ueq([SG_W, IG_A], Ls, Rs)
+        n = 2
+        sM = F[3 * n + 4]
+        assert abs(sM - u) <= 1e-9 * (abs(u) + 1)
+        up = Ls if u >= 0 else Rs
+        assert abs(F[0] - up[0] * u) <= 1e-9 * (abs(u) + 1e-12)
+        assert abs(F[3 * n] - (up[8] * u * u + 1e5)) <= 1e-9 * 1e5
+
+
+# --------------------------------------------------------------------------------------------
+# Relaxation
+# --------------------------------------------------------------------------------------------
+
+def test_relaxation_invariants_and_bisection_root():
+    rng = np.random.default_rng(6)
+    eos = (po.OrcEos * 3)(eos_struct(*SG_W), eos_struct(1, 2.43, 5.3e9, 1800.0, 0.0), eos_struct(*IG_A))
+    for n in (2, 3):
+        for _ in range(200):
+            a = rng.dirichlet(np.ones(n))
+            a = np.clip(a, 1e-5, None)
+            a /= a.sum()
+            r = np.array([1000.0, 1185.0, 1.2][:n]) * 10 ** rng.uniform(-0.05, 0.05, size=n)
+            p = 10 ** rng.uniform(5, 9, size=n)
+            a0, r0, p0 = a.copy(), r.copy(), p.copy()
+            ps = np.zeros(1)
+            it = L.orc_relax_cell(n, eos, 0.0, 0, a.ctypes.data_as(DP), r.ctypes.data_as(DP),
+                                  p.ctypes.data_as(DP), ps.ctypes.data_as(DP))
+            assert 0 < it < 100
+            assert abs(a.sum() - 1.0) <= 1.1e-10
+            assert np.all(p == ps[0])
+            assert np.allclose(a * r, a0 * r0, rtol=1e-13, atol=0)
+
+            # bisection on f(p) = sum a0 r0 / rho*_k(p) - 1 over the admissible range
+            def f(x):
+                return sum(a0[k] * r0[k] / L.orc_eos_rho_pfinal(C.byref(eos[k]), 0.0, p0[k], r0[k], x, None)
+                           for k in range(n)) - 1.0
+            lo, hi = 1e-8 + 1.0, 1e12
+            flo, fhi = f(lo), f(hi)
+            assert flo * fhi < 0
+            for _ in range(200):
+                mid = 0.5 * (lo + hi)
+                fm = f(mid)
+                if fm * flo > 0:
+                    lo, flo = mid, fm
+                else:
+                    hi = mid
+            # the Newton stop is |sum alpha - 1| <= 1e-10, i.e. |p - root| <= 1e-10 / |f'(root)|
+            root = 0.5 * (lo + hi)
+            h = 1e-6 * root
+            fp = abs(f(root + h) - f(root - h)) / (2 * h)
+            assert abs(f(ps[0])) <= 1.1e-10
+            assert abs(ps[0] - root) <= 1.1e-10 / fp + 1e-12 * root
+
+
+# --------------------------------------------------------------------------------------------
+# Full steps
+# --------------------------------------------------------------------------------------------
+
+def _totals(case, st):
+    """Cell integrals of mass per phase, momentum and total energy (Cell.cpp:863-895)."""
+    nph = len(case["eos"])
+    D = (case["nx"] > 1) + (case["ny"] > 1) + (case["nz"] > 1)
+    if case["model"] == "euler":
+        rho = st[0]
+        mom = [rho * st[2 + d] for d in range(D)]
+        return np.array([rho.sum()] + [m.sum() for m in mom] + [(rho * st[-1]).sum()])
+    m = [st[3 * k] * st[3 * k + 1] for k in range(nph)]
+    rho = sum(m)
+    mom = [rho * st[3 * nph + d] for d in range(D)]
+    return np.array([mk.sum() for mk in m] + [x.sum() for x in mom] + [(rho * st[-1]).sum()])
+
+
+def _quiet_case(model, order, nph=2, D=2):
+    from ecogen_amd.case import IG_AIR, SG_WATER, SG_EPOXY
+    e = 1e-6
+    if model == "euler":
+        doms = [dict(kind="entire", rho=[1.0], p=[1.e5], vel=[0., 0., 0.]),
+                dict(kind="sphere" if D == 3 else "disc", center=(0.5, 0.5, 0.5), radius=0.12, rho=[3.0], p=[3.e5],
+                     vel=[0., 0., 0.])]
+        eos = [dict(IG_AIR)]
+    else:
+        eos = [dict(SG_WATER), dict(IG_AIR), dict(SG_EPOXY)][:nph]
+        base = [1 - (nph - 1) * e] + [e] * (nph - 1)
+        bub = [e] * (nph - 1) + [1 - (nph - 1) * e]
+        rho = [1000., 1.2, 1185.][:nph]
+        doms = [dict(kind="entire", alpha=base, rho=rho, p=[1e5] * nph, vel=[0., 0., 0.]),
+                dict(kind="sphere" if D == 3 else "disc", center=(0.5, 0.5, 0.5), radius=0.12, alpha=bub, rho=rho,
+                     p=[1e5] * nph, vel=[0., 0., 0.]),
+                dict(kind="sphere" if D == 3 else "disc", center=(0.5, 0.5, 0.5), radius=0.06, alpha=base,
+                     rho=[1100., 1.2, 1185.][:nph], p=[3e8] * nph, vel=[0., 0., 0.])]
+    n = 32 if D == 2 else 20
+    return dict(nx=n, ny=n, nz=1 if D == 2 else n, lx=1.0, ly=1.0, lz=1.0, model=model, eos=eos, order=order,
+                limiters=("minmod",) * 4 if order == 2 else ("none",) * 4, cfl=0.8, relax_p=(model != "euler"),
+                bc=("nonreflecting",) * 6, domains=doms)
+
+
+@pytest.mark.parametrize("model,order,nph,D", [("euler", 1, 1, 2), ("euler", 2, 1, 3), ("pueq", 1, 2, 2),
+                                               ("pueq", 2, 2, 2), ("ueq", 1, 3, 3), ("ueq", 2, 2, 2)])
+def test_conservation_over_steps(model, order, nph, D):
+    """Boundaries stay quiescent for the steps taken: the integrals are conserved to round-off."""
+    case = _quiet_case(model, order, nph, D)
+    o = Oracle(case)
+    s0 = o.get_state()
+    t0 = _totals(case, s0)
+    dt = 1e-15
+    for _ in range(5):
+        dt = case["cfl"] * o.step(dt)
+    s1 = o.get_state()
+    t1 = _totals(case, s1)
+    # precondition: the boundary layers of cells have not moved yet (beyond round-off)
+    shp = (case["nz"], case["ny"], case["nx"])
+    a0, a1 = s0[0].reshape(shp), s1[0].reshape(shp)
+    for sl in ((slice(None), slice(None), 0), (slice(None), slice(None), -1), (slice(None), 0), (slice(None), -1)):
+        assert np.max(np.abs(a1[sl] - a0[sl])) <= 1e-12 * np.max(np.abs(a0[sl]))
+    scale = np.abs(t0) + 1e-30
+    mom_scale = max(abs(t0[-1]), 1.0) ** 0.5 * 1e-6
+    nmass = 1 if model == "euler" else nph
+    assert np.all(np.abs(t1[:nmass] - t0[:nmass]) <= 1e-13 * scale[:nmass])
+    assert np.all(np.abs(t1[nmass:-1] - t0[nmass:-1]) <= mom_scale)
+    assert abs(t1[-1] - t0[-1]) <= 1e-13 * scale[-1]
+
+
+@pytest.mark.parametrize("model,order", [("pueq", 1), ("pueq", 2), ("ueq", 1), ("ueq", 2)])
+def test_interface_at_rest_stays_at_rest(model, order):
+    """Abgrall's criterion: a material interface in uniform p and u keeps p and u uniform."""
+    from ecogen_amd.case import IG_AIR, SG_WATER
+    e = 1e-6
+    case = dict(nx=60, ny=1, nz=1, lx=1.0, ly=0.1, lz=0.1, model=model, eos=[dict(SG_WATER), dict(IG_AIR)],
+                order=order, limiters=("minmod",) * 4 if order == 2 else ("none",) * 4, cfl=0.8, relax_p=True,
+                domains=[dict(kind="entire", alpha=[1 - e, e], rho=[1000., 1.2], p=[1e5, 1e5], vel=[50., 0., 0.]),
+                         dict(kind="halfspace", axis=0, origin=0.4, dir=1, alpha=[e, 1 - e], rho=[1000., 1.2],
+                              p=[1e5, 1e5], vel=[50., 0., 0.])])
+    o = Oracle(case)
+    dt = 1e-15
+    for _ in range(30):
+        dt = case["cfl"] * o.step(dt)
+    st = o.get_state()
+    assert np.max(np.abs(st[2] - 1e5)) <= 1e-6 * 1e5
+    assert np.max(np.abs(st[5] - 1e5)) <= 1e-6 * 1e5
+    assert np.max(np.abs(st[6] - 50.0)) <= 1e-8 * 50.0
+
+
+def test_ueq_scheme_correction_closes_energy():
+    """VELOCITYEQ without relaxation: after a step sum_k alpha_k rho_k e_k = rho (E - u^2/2) per cell
+    (FluxUEq::schemeCorrection distributes the total-energy error over the phases)."""
+    case = _quiet_case("ueq", 1, 2, 2)
+    case["relax_p"] = False
+    o = Oracle(case)
+    dt = 1e-15
+    for _ in range(3):
+        dt = case["cfl"] * o.step(dt)
+    st = o.get_state()
+    g = [4.4, 1.4]
+    pinf = [6e8, 0.0]
+    rhoe = sum(st[3 * k] * (st[3 * k + 2] + g[k] * pinf[k]) / (g[k] - 1) for k in range(2))
+    rho = st[0] * st[1] + st[3] * st[4]
+    u, v = st[6], st[7]
+    mix = rho * (st[8] - 0.5 * (u * u + v * v))
+    assert np.max(np.abs(rhoe - mix) / np.abs(mix)) <= 1e-9
+
+
+def test_kapila_energy_correction_makes_mixture_eos_consistent():
+    case = _quiet_case("pueq", 1, 2, 2)
+    o = Oracle(case)
+    dt = 1e-15
+    for _ in range(4):
+        dt = case["cfl"] * o.step(dt)
+    st = o.get_state()
+    assert np.array_equal(st[2], st[5])  # one pressure
+    g = [4.4, 1.4]
+    pinf = [6e8, 0.0]
+    rhoe = sum(st[3 * k] * (st[3 * k + 2] + g[k] * pinf[k]) / (g[k] - 1) for k in range(2))
+    rho = st[0] * st[1] + st[3] * st[4]
+    mix = rho * (st[8] - 0.5 * (st[6] ** 2 + st[7] ** 2))
+    assert np.max(np.abs(rhoe - mix) / np.abs(mix)) <= 1e-9
+
+
+# --------------------------------------------------------------------------------------------
+# Accuracy
+# --------------------------------------------------------------------------------------------
+
+def _exact_sod_density(x, t, gam=1.4, rl=1.0, pl=1e5, rr=0.125, pr=1e4, x0=0.5):
+    """Exact Riemann solution (ideal gas, both sides at rest), density profile."""
+    from scipy.optimize import brentq
+    cl, cr = np.sqrt(gam * pl / rl), np.sqrt(gam * pr / rr)
+
+    def fk(p, r, pk, ck):
+        if p > pk:
+            A, B = 2 / ((gam + 1) * r), (gam - 1) / (gam + 1) * pk
+            return (p - pk) * np.sqrt(A / (p + B))
+        return 2 * ck / (gam - 1) * ((p / pk) ** ((gam - 1) / (2 * gam)) - 1)
+
+    ps = brentq(lambda p: fk(p, rl, pl, cl) + fk(p, rr, pr, cr), 1e-6, 1e7)
+    us = 0.5 * (fk(ps, rr, pr, cr) - fk(ps, rl, pl, cl))
+    rsl = rl * (ps / pl) ** (1 / gam)
+    rsr = rr * ((ps / pr + (gam - 1) / (gam + 1)) / ((gam - 1) / (gam + 1) * ps / pr + 1))
+    sshock = (rsr * us) / (rsr - rr)
+    csl = cl * (ps / pl) ** ((gam - 1) / (2 * gam))
+    xi = (x - x0) / t
+    out = np.empty_like(x)
+    for i, s in enumerate(xi):
+        if s < -cl:
+            out[i] = rl
+        elif s < us - csl:
+            c = 2 / (gam + 1) * (cl - (gam - 1) / 2 * s)
+            out[i] = rl * (c / cl) ** (2 / (gam - 1))
+        elif s < us:
+            out[i] = rsl
+        elif s < sshock:
+            out[i] = rsr
+        else:
+            out[i] = rr
+    return out
+
+
+def test_sod_converges_to_exact_solution():
+    from ecogen_amd.case import sod_case
+    errs = []
+    for order in (1, 2):
+        for n in (200, 800):
+            c = sod_case(n)
+            c["order"] = order
+            if order == 2:
+                c["limiters"] = ("minmod",) * 4
+            o = Oracle(c)
+            t, _, _ = o.run(0, 5e-4)
+            x = (np.arange(n) + 0.5) / n
+            errs.append(np.mean(np.abs(o.get_state()[0] - _exact_sod_density(x, t))))
+    e1c, e1f, e2c, e2f = errs
+    assert e1f < 0.6 * e1c and e2f < 0.6 * e2c      # converges
+    assert e2c < e1c and e2f < e1f                  # MUSCL beats Godunov
+    assert e1f < 6e-3 and e2f < 3e-3
+
+
+def test_muscl_second_order_on_smooth_wave():
+    """Smooth acoustic pulse: the midpoint MUSCL error falls about 4x per refinement, Godunov about 2x."""
+    def run(n, order):
+        xs = (np.arange(n) + 0.5) / n
+        c = dict(nx=n, ny=1, nz=1, lx=1.0, ly=0.1, lz=0.1, model="euler", eos=[dict(type="IG", gamma=1.4, cv=800.)],
+                 order=order, limiters=("mc",) * 4 if order == 2 else ("none",) * 4, cfl=0.5,
+                 domains=[dict(kind="entire", rho=[1.0], p=[1e5], vel=[0., 0., 0.])])
+        o = Oracle(c)
+        st = o.get_state()
+        amp = 1e-3 * np.exp(-((xs - 0.5) / 0.08) ** 2)
+        st[0] = 1.0 + amp
+        st[1] = 1e5 * (1.0 + 1.4 * amp)
+        st[-1] = st[1] / (0.4 * st[0])
+        o.set_state(st)
+        o.run(0, 2e-4)
+        return xs, o.get_state()[0]
+    ref_x, ref = run(3200, 2)
+    errs = {}
+    for order in (1, 2):
+        e = []
+        for n in (100, 200, 400):
+            x, r = run(n, order)
+            rr = ref.reshape(n, -1).mean(axis=1)
+            e.append(np.mean(np.abs(r - rr)))
+        errs[order] = e
+    r1 = np.log2(errs[1][1] / errs[1][2])
+    r2 = np.log2(errs[2][1] / errs[2][2])
+    assert 0.6 < r1 < 1.4
+    assert r2 > 1.6
