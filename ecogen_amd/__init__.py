@@ -74,7 +74,11 @@ SIGNATURES = {
     "ecg_synchronize": (C.c_int, [C.c_void_p, C.POINTER(EcgStepStats)]),
     "ecg_reset_clock": (C.c_int, [C.c_void_p]),
     "ecg_stream": (C.c_void_p, [C.c_void_p]),
+    "ecg_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "ecg_kernel_times": (C.c_int, [C.c_void_p, _DP, C.POINTER(C.c_int64), C.c_int]),
+    "ecg_calibration_copy": (C.c_int, [C.c_void_p, C.c_int64]),
 }
+KERNEL_KINDS = ("step", "predict", "correct", "clock")
 
 
 def library(path=None):
@@ -209,6 +213,19 @@ class Solver:
     def synchronize(self):
         _check(self.L.ecg_synchronize(self.h, C.byref(self.stats)))
         return self.stats.as_dict()
+
+    def calibration_copy(self, nbytes):
+        _check(self.L.ecg_calibration_copy(self.h, int(nbytes)))
+
+    def set_profiling(self, on=True):
+        _check(self.L.ecg_set_profiling(self.h, int(bool(on))))
+
+    def kernel_times(self):
+        """{kind: (total ms, launches)} since profiling was switched on."""
+        ms = np.zeros(4)
+        n = (C.c_int64 * 4)()
+        _check(self.L.ecg_kernel_times(self.h, ms.ctypes.data_as(_DP), n, 4))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNEL_KINDS)}
 
     @property
     def stream(self):

@@ -55,6 +55,12 @@ __global__ void k_clock_set(Clock* clk, Clock v) {
   if (threadIdx.x == 0) *clk = v;
 }
 
+// Plain streaming copy (16 bytes per lane per iteration), used to calibrate byte counters.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, int64_t n2) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
+}
+
 void launch_clock(Clock* clk, double cfl, hipStream_t s) { hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, clk, cfl); }
 
 }  // namespace ecg
@@ -91,7 +97,51 @@ struct ecg_ctx {
   fill_fn fill = nullptr;
   int64_t pending = 0;                         // steps enqueued since the clock was last read
   int cur_at_sync = 0;
+  // Kernel timing (ecg_set_profiling): event pairs per launch, resolved at synchronisation.
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  double kt_ms[ECG_K_KINDS] = {0, 0, 0, 0};
+  int64_t kt_n[ECG_K_KINDS] = {0, 0, 0, 0};
 };
+
+static hipEvent_t take_event(ecg_ctx* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+// Launch f() between two recorded events when profiling.
+template <class F>
+static void timed(ecg_ctx* c, int kind, F f) {
+  if (!c->profiling) {
+    f();
+    return;
+  }
+  hipEvent_t a = take_event(c), b = take_event(c);
+  hipEventRecord(a, c->stream);
+  f();
+  hipEventRecord(b, c->stream);
+  c->ev_pending.push_back({kind, {a, b}});
+}
+
+static void resolve_events(ecg_ctx* c) {
+  for (auto& r : c->ev_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.second.first, r.second.second) == hipSuccess) {
+      c->kt_ms[r.first] += ms;
+      c->kt_n[r.first] += 1;
+    }
+    c->ev_pool.push_back(r.second.first);
+    c->ev_pool.push_back(r.second.second);
+  }
+  c->ev_pending.clear();
+}
 
 extern "C" {
 
@@ -267,6 +317,8 @@ void ecg_destroy(ecg_ctx* c) {
   if (c->cnt) hipFree(c->cnt);
   if (c->hclk) hipHostFree(c->hclk);
   if (c->hcnt) hipHostFree(c->hcnt);
+  for (auto& r : c->ev_pending) { c->ev_pool.push_back(r.second.first); c->ev_pool.push_back(r.second.second); }
+  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -371,14 +423,14 @@ static int enqueue_one(ecg_ctx* c) {
   StepLaunch L;
   if (c->cfg.order == 1) {
     L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[nxt]; L.scale_half = 0; L.stage = 0;
-    c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads);
+    timed(c, ECG_K_STEP, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads); });
   } else {
     L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[2]; L.scale_half = 1; L.stage = 1;
-    c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads);
+    timed(c, ECG_K_PREDICT, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads); });
     L.wn = c->W[c->cur]; L.wsrc = c->W[2]; L.wout = c->W[nxt]; L.scale_half = 0; L.stage = 2;
-    c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads);
+    timed(c, ECG_K_CORRECT, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads); });
   }
-  launch_clock(c->clk, c->cfg.cfl, c->stream);
+  timed(c, ECG_K_CLOCK, [&] { launch_clock(c->clk, c->cfg.cfl, c->stream); });
   HIPCHK(hipGetLastError());
   c->cur = nxt;
   c->pending += 1;
@@ -389,6 +441,7 @@ static int read_clock(ecg_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->hclk, c->clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  resolve_events(c);
   return ECG_OK;
 }
 
@@ -484,6 +537,41 @@ int ecg_synchronize(ecg_ctx* c, ecg_step_stats* st) {
   if (rc != ECG_OK) return rc;
   c->pending = 0;
   fill_stats(c, st);
+  return ECG_OK;
+}
+
+int ecg_calibration_copy(ecg_ctx* c, int64_t nbytes) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  const int64_t cap = (int64_t)sizeof(double) * c->pitch * c->nvar;
+  if (nbytes <= 0 || nbytes > cap || nbytes % 16) return fail(ECG_E_ARG, "nbytes must be a positive multiple of 16 "
+                                                                       "not larger than one state buffer");
+  const int64_t n2 = nbytes / 16;
+  const int threads = 256;
+  const int blocks = (int)std::min<int64_t>((n2 + threads - 1) / threads, 256 * 1024);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(threads), 0, c->stream, (const double2*)c->W[c->cur],
+                     (double2*)c->W[1 - c->cur], n2);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return ECG_OK;
+}
+
+int ecg_set_profiling(ecg_ctx* c, int on) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  int rc = read_clock(c);
+  if (rc != ECG_OK) return rc;
+  c->profiling = (on != 0);
+  for (int k = 0; k < ECG_K_KINDS; k++) { c->kt_ms[k] = 0.; c->kt_n[k] = 0; }
+  return ECG_OK;
+}
+
+int ecg_kernel_times(ecg_ctx* c, double* ms_total, int64_t* launches, int nkinds) {
+  if (!c || nkinds < 0) return fail(ECG_E_ARG, "bad arguments");
+  int rc = read_clock(c);
+  if (rc != ECG_OK) return rc;
+  for (int k = 0; k < nkinds && k < ECG_K_KINDS; k++) {
+    if (ms_total) ms_total[k] = c->kt_ms[k];
+    if (launches) launches[k] = c->kt_n[k];
+  }
   return ECG_OK;
 }
 

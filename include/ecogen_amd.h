@@ -143,6 +143,20 @@ int ecg_reset_clock(ecg_ctx* ctx);
 /* Stream the context runs on (a hipStream_t), for event timing by the caller. */
 void* ecg_stream(ecg_ctx* ctx);
 
+/* Per-kernel timing with HIP events recorded around every launch on the
+ * context's stream.  Kinds: 0 first-order step, 1 second-order predictor,
+ * 2 second-order corrector, 3 clock update.  ecg_kernel_times accumulates the
+ * elapsed milliseconds and launch counts since profiling was switched on. */
+enum { ECG_K_STEP = 0, ECG_K_PREDICT = 1, ECG_K_CORRECT = 2, ECG_K_CLOCK = 3, ECG_K_KINDS = 4 };
+int ecg_set_profiling(ecg_ctx* ctx, int on);
+int ecg_kernel_times(ecg_ctx* ctx, double* ms_total, int64_t* launches, int nkinds);
+
+/* Byte-counter calibration: one launch of a plain streaming copy kernel that reads
+ * and writes nbytes (at most one state buffer) between the context's two state
+ * buffers.  Profilers divide a kernel's FETCH_SIZE / WRITE_SIZE by what they
+ * report for this launch to turn counter units into bytes. */
+int ecg_calibration_copy(ecg_ctx* ctx, int64_t nbytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,66 @@
+"""C ABI of libecogen_amd.so, checked without a GPU.
+
+The library must load, export every function include/ecogen_amd.h declares
+(with the ctypes signatures the Python host binding uses), and reject invalid
+configurations with the reference's error messages before touching the device.
+"""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import ecogen_amd as ea
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ecogen_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ecg_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = ea.library()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in the header but not exported"
+        assert n in ea.SIGNATURES, f"{n} has no ctypes signature in the host binding"
+
+
+@pytest.mark.parametrize("cname,pyclass", [("ecg_config", ea.EcgConfig), ("ecg_domain", ea.EcgDomain),
+                                           ("ecg_eos", ea.EcgEos), ("ecg_step_stats", ea.EcgStepStats)])
+def test_struct_layouts_match_header_field_order(cname, pyclass):
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    blocks = dict((name, body) for body, name in re.findall(r"typedef struct \{([^{}]*)\} (\w+);", text))
+    fields = re.findall(r"([a-z_0-9]+)(?:\[[A-Z_0-9]+\])?\s*[;,]", blocks[cname])
+    assert fields == [f for f, _ in pyclass._fields_]
+
+
+def _case(**kw):
+    c = dict(nx=8, ny=1, nz=1, lx=1.0, ly=1.0, lz=1.0, model="euler", eos=[dict(type="IG", gamma=1.4)], order=1)
+    c.update(kw)
+    return c
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(nx=1), "Number of cells in the x-direction has to be superior to 1"),
+    (dict(ny=1, nz=4), "z-direction"),
+    (dict(model="pueq"), "2 or 3 phases"),
+    (dict(eos=[dict(type="IG", gamma=0.9)]), "invalid EOS"),
+    (dict(order=2, limiters=("none",) * 4), "limiters"),
+    (dict(order=2, limiters=("minmod",) * 4, bc=("wall",) * 6), "not built"),
+])
+def test_invalid_configurations_are_rejected(kw, msg):
+    lib = ea.library()
+    cfg = ea.make_config(_case(**kw))
+    h = lib.ecg_create(C.byref(cfg))
+    assert not h
+    assert msg in lib.ecg_last_error().decode()
+
+
+def test_version_string():
+    assert b"gfx950" in ea.library().ecg_version()

@@ -423,7 +423,7 @@ def _quiet_case(model, order, nph=2, D=2):
                      p=[1e5] * nph, vel=[0., 0., 0.]),
                 dict(kind="sphere" if D == 3 else "disc", center=(0.5, 0.5, 0.5), radius=0.06, alpha=base,
                      rho=[1100., 1.2, 1185.][:nph], p=[3e8] * nph, vel=[0., 0., 0.])]
-    n = 32 if D == 2 else 20
+    n = 40 if D == 2 else 28
     return dict(nx=n, ny=n, nz=1 if D == 2 else n, lx=1.0, ly=1.0, lz=1.0, model=model, eos=eos, order=order,
                 limiters=("minmod",) * 4 if order == 2 else ("none",) * 4, cfl=0.8, relax_p=(model != "euler"),
                 bc=("nonreflecting",) * 6, domains=doms)
