@@ -60,14 +60,16 @@ def traffic_from_profiles(kernel):
 
 WORKLOADS = {
     "c5": "3D two-phase VELOCITYEQ water/air {n}^3, {extra} air pockets (r 2-10 cells), 5e8 Pa slab drive, "
-          "MUSCL minmod + explicit midpoint, infinite P-relaxation, plain FP64",
+          "MUSCL minmod + explicit midpoint, infinite P-relaxation",
     "c4": "3D three-phase VELOCITYEQ water/epoxy/air {n}^3, {extra} inclusions, 1e9 Pa water shock, first order, "
-          "infinite P-relaxation, plain FP64",
+          "infinite P-relaxation",
     "c3": "2D Kapila (PRESSUREVELOCITYEQ) water/air {n}^2, {extra} bubbles, 1e9 Pa water shock, MUSCL minmod + "
-          "explicit midpoint, plain FP64",
-    "c2": "2D Kapila (PRESSUREVELOCITYEQ) water/air {n}^2, {extra} bubbles, 1e9 Pa water shock, first order, "
-          "plain FP64",
+          "explicit midpoint",
+    "c2": "2D Kapila (PRESSUREVELOCITYEQ) water/air {n}^2, {extra} bubbles, 1e9 Pa water shock, first order",
 }
+FORMS = {"fp64": "plain FP64",
+         "fp32": "FP32 state, reconstruction and fluxes (compensated sums); FP64 cell update, EOS inversions "
+                 "and relaxation Newton"}
 DEFAULT_N = {"c5": 512, "c4": 256, "c3": 2048, "c2": 2048}
 
 
@@ -110,6 +112,8 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="cells per axis (default: the workload's size)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate", action="store_true", help="also launch the byte-counter calibration copy")
+    ap.add_argument("--form", choices=("fp64", "fp32"), default="fp64",
+                    help="fp64: plain FP64; fp32: FP32 storage/fluxes with FP64 update and relaxation")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition is "
@@ -121,6 +125,8 @@ def main():
 
     n = args.n or DEFAULT_N[args.workload]
     case, npk = make_case(args.workload, n)
+    case["precision"] = args.form
+    esize = 8 if args.form == "fp64" else 4
     print(f"[bench] building {case['name']} ({args.workload}, {npk} inclusions)", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     s = Solver(case)
@@ -151,7 +157,7 @@ def main():
     st = s.synchronize()
     kt = s.kernel_times()
     if args.calibrate:
-        nb = 8 * nvar * ncell // 16 * 16
+        nb = esize * nvar * ncell // 16 * 16
         s.calibration_copy(nb)
         print(f"[bench] calibration copy: {nb} bytes read and {nb} bytes written in one launch", file=sys.stderr,
               flush=True)
@@ -167,11 +173,11 @@ def main():
     kind = max(("predict", "correct", "step"), key=lambda k: kt[k][0])
     kms, kn = kt[kind]
     avg_s = kms / kn / 1e3
-    bytes_launch = transfers[kind] * nvar * 8 * ncell
+    bytes_launch = transfers[kind] * nvar * esize * ncell
     achieved = bytes_launch / avg_s / 1e9
     kernel_name = {"predict": "k_step (second order, predictor)", "correct": "k_step (second order, corrector)",
                    "step": "k_step (first order)"}[kind]
-    traffic = traffic_from_profiles(f"{args.workload}:{kind}")
+    traffic = traffic_from_profiles(f"{args.workload}:{args.form}:{kind}")
     out = {
         "metric": "Mcell-updates/s: plain FP64 vs FP32+learned-relax at stated L1 err; HBM GB/s %",
         "value": round(value, 3),
@@ -183,14 +189,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.form == "fp64" else "f32+f64",
         "data": "synthetic (seeded random air pockets, numpy default_rng 20261202)",
         "config": {
             "workload": WORKLOADS[args.workload].format(n=n, extra=npk),
             "cells": ncell,
             "state_vars": nvar,
-            "form": "plain FP64",
-            "bytes_per_cell_update": (5 if case["order"] == 2 else 2) * nvar * 8,
+            "form": FORMS[args.form],
+            "bytes_per_cell_update": (5 if case["order"] == 2 else 2) * nvar * esize,
         },
         "roofline": {
             "bound": "hbm",
@@ -204,7 +210,7 @@ def main():
             "algorithmic_bytes_per_launch": bytes_launch,
         },
         "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items() if v[1]},
-        "whole_step_hbm_gbs": round((5 if case["order"] == 2 else 2) * nvar * 8 * value * 1e6 / 1e9, 2),
+        "whole_step_hbm_gbs": round((5 if case["order"] == 2 else 2) * nvar * esize * value * 1e6 / 1e9, 2),
     }
     if not args.no_cpu_baseline:
         print("[bench] timing the oracle on the host (bounded replica)", file=sys.stderr, flush=True)

@@ -12,25 +12,38 @@
 #include "ecg_kernels.hpp"
 
 namespace ecg {
-step_fn get_step_fn_euler(int nph, int dim);
-step_fn get_step_fn_pueq(int nph, int dim);
-step_fn get_step_fn_ueq(int nph, int dim);
-fill_fn get_fill_fn_euler(int nph, int dim);
-fill_fn get_fill_fn_pueq(int nph, int dim);
-fill_fn get_fill_fn_ueq(int nph, int dim);
+step_fn get_step_fn_euler_f64(int nph, int dim);
+step_fn get_step_fn_pueq_f64(int nph, int dim);
+step_fn get_step_fn_ueq_f64(int nph, int dim);
+step_fn get_step_fn_euler_f32(int nph, int dim);
+step_fn get_step_fn_pueq_f32(int nph, int dim);
+step_fn get_step_fn_ueq_f32(int nph, int dim);
+fill_fn get_fill_fn_euler_f64(int nph, int dim);
+fill_fn get_fill_fn_pueq_f64(int nph, int dim);
+fill_fn get_fill_fn_ueq_f64(int nph, int dim);
+fill_fn get_fill_fn_euler_f32(int nph, int dim);
+fill_fn get_fill_fn_pueq_f32(int nph, int dim);
+fill_fn get_fill_fn_ueq_f32(int nph, int dim);
 
-step_fn get_step_fn(int model, int nph, int dim, int /*order*/) {
-  if (model == M_EULER) return get_step_fn_euler(nph, dim);
-  if (model == M_PUEQ) return get_step_fn_pueq(nph, dim);
-  if (model == M_UEQ) return get_step_fn_ueq(nph, dim);
+static step_fn get_step_fn(int model, int nph, int dim, bool f32) {
+  if (model == M_EULER) return f32 ? get_step_fn_euler_f32(nph, dim) : get_step_fn_euler_f64(nph, dim);
+  if (model == M_PUEQ) return f32 ? get_step_fn_pueq_f32(nph, dim) : get_step_fn_pueq_f64(nph, dim);
+  if (model == M_UEQ) return f32 ? get_step_fn_ueq_f32(nph, dim) : get_step_fn_ueq_f64(nph, dim);
   return nullptr;
 }
 
-fill_fn get_fill_fn(int model, int nph, int dim) {
-  if (model == M_EULER) return get_fill_fn_euler(nph, dim);
-  if (model == M_PUEQ) return get_fill_fn_pueq(nph, dim);
-  if (model == M_UEQ) return get_fill_fn_ueq(nph, dim);
+static fill_fn get_fill_fn(int model, int nph, int dim, bool f32) {
+  if (model == M_EULER) return f32 ? get_fill_fn_euler_f32(nph, dim) : get_fill_fn_euler_f64(nph, dim);
+  if (model == M_PUEQ) return f32 ? get_fill_fn_pueq_f32(nph, dim) : get_fill_fn_pueq_f64(nph, dim);
+  if (model == M_UEQ) return f32 ? get_fill_fn_ueq_f32(nph, dim) : get_fill_fn_ueq_f64(nph, dim);
   return nullptr;
+}
+
+// Cells per 256-thread tile (must match Tile<D> in ecg_step_impl.hpp).
+void tile_extent(int dim, int t[3]) {
+  if (dim == 1) { t[0] = 256; t[1] = 1; t[2] = 1; }
+  else if (dim == 2) { t[0] = 16; t[1] = 16; t[2] = 1; }
+  else { t[0] = 16; t[1] = 4; t[2] = 4; }
 }
 
 // Run::solver bookkeeping after a step (Run.cpp:417-486): t += dt, dt = cfl*dtMax.
@@ -86,7 +99,8 @@ struct ecg_ctx {
   int D = 1, nvar = 0;
   int64_t ncell = 0, pitch = 0;
   hipStream_t stream = nullptr;
-  double* W[3] = {nullptr, nullptr, nullptr};  // two time levels + second-order predicted state
+  void* W[3] = {nullptr, nullptr, nullptr};    // two time levels + second-order predicted state
+  int esize = 8;                               // bytes per state element (8: FP64 form, 4: FP32 forms)
   int cur = 0;                                 // W[cur] holds the current state
   double* tables = nullptr;
   Clock* clk = nullptr;                        // device clock
@@ -199,7 +213,8 @@ static int check_config(const ecg_config* c) {
       return fail(ECG_E_UNSUPPORTED, "second-order wall/symmetry boundaries (BoundCondWallO2Cartesian) are not built");
   }
   if (!(c->cfl > 0)) return fail(ECG_E_ARG, "CFL must be positive");
-  if (c->precision != ECG_FP64) return fail(ECG_E_UNSUPPORTED, "this build runs the FP64 path only");
+  if (c->precision != ECG_FP64 && c->precision != ECG_FP32_RELAX64)
+    return fail(ECG_E_UNSUPPORTED, "learned warm start is not available in this build");
   return ECG_OK;
 }
 
@@ -214,8 +229,10 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   c->ncell = (int64_t)g.nx * g.ny * g.nz;
   c->pitch = (c->ncell + 63) / 64 * 64;
   c->nvar = (g.model == ECG_EULER) ? (2 + c->D + 1) : (3 * g.nph + c->D + 1);
-  c->step = get_step_fn(g.model, g.nph, c->D, g.order);
-  c->fill = get_fill_fn(g.model, g.nph, c->D);
+  const bool f32 = (g.precision != ECG_FP64);
+  c->esize = f32 ? 4 : 8;
+  c->step = get_step_fn(g.model, g.nph, c->D, f32);
+  c->fill = get_fill_fn(g.model, g.nph, c->D, f32);
   if (!c->step || !c->fill) {
     fail(ECG_E_UNSUPPORTED, "no kernel built for this model/phase/dimension combination");
     delete c;
@@ -252,6 +269,9 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   for (int i = 0; i < 6; i++) P.bc[i] = g.bc[i];
   for (int i = 0; i < 4; i++) P.lim[i] = g.lim[i];
   for (int k = 0; k < g.nph; k++) P.eos[k] = make_eos(g.eos[k]);
+  int tex[3];
+  tile_extent(c->D, tex);
+  for (int a = 0; a < 3; a++) P.tiles[a] = (n3[a] + tex[a] - 1) / tex[a];
 
   // Distance tables of the interior faces (Element.cpp:77-81, Face.cpp:52-56).
   std::vector<double> tab;
@@ -278,7 +298,7 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
     }
   };
   chk(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-  const size_t bytes = sizeof(double) * (size_t)c->pitch * c->nvar;
+  const size_t bytes = (size_t)c->esize * (size_t)c->pitch * c->nvar;
   const int nbuf = (g.order == 2) ? 3 : 2;
   for (int b = 0; b < nbuf && ok; b++) chk(hipMalloc(&c->W[b], bytes), "hipMalloc(state)");
   if (ok) chk(hipMalloc(&c->tables, sizeof(double) * tab.size()), "hipMalloc(tables)");
@@ -376,9 +396,7 @@ int ecg_fill(ecg_ctx* c, const ecg_domain* doms, int ndom) {
   DomainDev* ddev = nullptr;
   HIPCHK(hipMalloc(&ddev, sizeof(DomainDev) * ndom));
   HIPCHK(hipMemcpyAsync(ddev, dd.data(), sizeof(DomainDev) * ndom, hipMemcpyHostToDevice, c->stream));
-  const int threads = 256;
-  const int blocks = (int)((c->ncell + threads - 1) / threads);
-  c->fill(c->P, ddev, ndom, c->W[c->cur], c->stream, blocks, threads);
+  c->fill(c->P, ddev, ndom, c->W[c->cur], c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipFree(ddev));
@@ -388,10 +406,18 @@ int ecg_fill(ecg_ctx* c, const ecg_domain* doms, int ndom) {
 int ecg_upload_state(ecg_ctx* c, const double* const* soa, int nvars) {
   if (!c || !soa) return fail(ECG_E_ARG, "null argument");
   if (nvars != c->nvar) return fail(ECG_E_ARG, "expected " + std::to_string(c->nvar) + " state arrays");
+  std::vector<float> tmp;
   for (int v = 0; v < nvars; v++) {
     if (!soa[v]) return fail(ECG_E_ARG, "null state array");
-    HIPCHK(hipMemcpyAsync(c->W[c->cur] + (size_t)v * c->pitch, soa[v], sizeof(double) * c->ncell,
-                          hipMemcpyHostToDevice, c->stream));
+    char* dst = (char*)c->W[c->cur] + (size_t)v * c->pitch * c->esize;
+    if (c->esize == 8) {
+      HIPCHK(hipMemcpyAsync(dst, soa[v], sizeof(double) * c->ncell, hipMemcpyHostToDevice, c->stream));
+    } else {
+      tmp.resize(c->ncell);
+      for (int64_t i = 0; i < c->ncell; i++) tmp[i] = (float)soa[v][i];
+      HIPCHK(hipMemcpyAsync(dst, tmp.data(), sizeof(float) * c->ncell, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   return ECG_OK;
@@ -404,12 +430,20 @@ int ecg_download_state(ecg_ctx* c, double* const* soa, int nvars) {
   if (nvars != c->nvar) return fail(ECG_E_ARG, "expected " + std::to_string(c->nvar) + " state arrays");
   int rc = settle(c);
   if (rc != ECG_OK) return rc;
+  std::vector<float> tmp;
   for (int v = 0; v < nvars; v++) {
     if (!soa[v]) return fail(ECG_E_ARG, "null state array");
-    HIPCHK(hipMemcpyAsync(soa[v], c->W[c->cur] + (size_t)v * c->pitch, sizeof(double) * c->ncell,
-                          hipMemcpyDeviceToHost, c->stream));
+    const char* src = (const char*)c->W[c->cur] + (size_t)v * c->pitch * c->esize;
+    if (c->esize == 8) {
+      HIPCHK(hipMemcpyAsync(soa[v], src, sizeof(double) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+      tmp.resize(c->ncell);
+      HIPCHK(hipMemcpyAsync(tmp.data(), src, sizeof(float) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      for (int64_t i = 0; i < c->ncell; i++) soa[v][i] = (double)tmp[i];
+    }
   }
-  HIPCHK(hipStreamSynchronize(c->stream));
   return ECG_OK;
 }
 
@@ -417,18 +451,16 @@ int ecg_download_state(ecg_ctx* c, double* const* soa, int nvars) {
 // marks done is skipped by every kernel, so the current buffer is settled from
 // the executed iteration count after synchronising.
 static int enqueue_one(ecg_ctx* c) {
-  const int threads = 256;
-  const int blocks = (int)((c->ncell + threads - 1) / threads);
   const int nxt = 1 - c->cur;
   StepLaunch L;
   if (c->cfg.order == 1) {
-    L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[nxt]; L.scale_half = 0; L.stage = 0;
-    timed(c, ECG_K_STEP, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads); });
+    L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[nxt]; L.stage = 0;
+    timed(c, ECG_K_STEP, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream); });
   } else {
-    L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[2]; L.scale_half = 1; L.stage = 1;
-    timed(c, ECG_K_PREDICT, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads); });
-    L.wn = c->W[c->cur]; L.wsrc = c->W[2]; L.wout = c->W[nxt]; L.scale_half = 0; L.stage = 2;
-    timed(c, ECG_K_CORRECT, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream, blocks, threads); });
+    L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[2]; L.stage = 1;
+    timed(c, ECG_K_PREDICT, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream); });
+    L.wn = c->W[c->cur]; L.wsrc = c->W[2]; L.wout = c->W[nxt]; L.stage = 2;
+    timed(c, ECG_K_CORRECT, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream); });
   }
   timed(c, ECG_K_CLOCK, [&] { launch_clock(c->clk, c->cfg.cfl, c->stream); });
   HIPCHK(hipGetLastError());
@@ -542,7 +574,7 @@ int ecg_synchronize(ecg_ctx* c, ecg_step_stats* st) {
 
 int ecg_calibration_copy(ecg_ctx* c, int64_t nbytes) {
   if (!c) return fail(ECG_E_ARG, "null context");
-  const int64_t cap = (int64_t)sizeof(double) * c->pitch * c->nvar;
+  const int64_t cap = (int64_t)c->esize * c->pitch * c->nvar;
   if (nbytes <= 0 || nbytes > cap || nbytes % 16) return fail(ECG_E_ARG, "nbytes must be a positive multiple of 16 "
                                                                        "not larger than one state buffer");
   const int64_t n2 = nbytes / 16;

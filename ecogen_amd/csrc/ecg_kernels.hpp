@@ -17,22 +17,28 @@ struct DomainDev {
   double vel[3];
 };
 
+// Buffers of one launch; the element type is the state type of the precision form.
 struct StepLaunch {
-  const double* wn;     // state at time n
-  const double* wsrc;   // state the fluxes are evaluated on (== wn for first order / first stage)
-  double* wout;         // written state
-  double scale_half;    // 1: the update uses 0.5*dt (second-order predictor)
+  const void* wn;       // state at time n
+  const void* wsrc;     // state the fluxes are evaluated on (== wn for first order / predictor)
+  void* wout;           // written state
   int stage;            // 0: first order; 1: predictor; 2: corrector
 };
 
-// Per (model, phases, dimension) launchers; defined in ecg_step_<model>.hip.
-typedef void (*step_fn)(const Params& P, const StepLaunch& L, Clock* clk, Counters* cnt, hipStream_t s,
-                        int blocks, int threads);
-typedef void (*fill_fn)(const Params& P, const DomainDev* doms, int ndom, double* w, hipStream_t s, int blocks,
-                        int threads);
+template <typename R>
+struct StepLaunchT {
+  const R* wn;
+  const R* wsrc;
+  R* wout;
+  int stage;
+};
 
-step_fn get_step_fn(int model, int nph, int dim, int order);
-fill_fn get_fill_fn(int model, int nph, int dim);
+// Per (model, phases, dimension, precision) launchers; defined in ecg_step_<model>_<precision>.hip.
+typedef void (*step_fn)(const Params& P, const StepLaunch& L, Clock* clk, Counters* cnt, hipStream_t s);
+typedef void (*fill_fn)(const Params& P, const DomainDev* doms, int ndom, void* w, hipStream_t s);
+
+// Tile extents (cells per block along x, y, z) for a problem dimension.
+void tile_extent(int dim, int t[3]);
 
 void launch_clock(Clock* clk, double cfl, hipStream_t s);
 

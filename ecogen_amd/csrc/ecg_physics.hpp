@@ -1,11 +1,12 @@
 // Device physics of the explicit finite-volume step (gfx950).
 //
-// Per-cell register formulation: a thread owns one cell, gathers the states of
-// its face neighbours, evaluates every face flux of the cell in the reference's
-// face-creation order and performs the cell update, relaxation and energy
-// correction in registers.  Every expression keeps the association order of
-// the reference expression it evaluates (built with -ffp-contract=off) so that
-// the FP64 path reproduces the reference's CPU arithmetic.
+// Face work (EOS evaluations, MUSCL reconstruction, Riemann solvers) is
+// templated on the arithmetic type R: double for the plain FP64 path, float
+// for the FP32 form.  The cell update (conservative variables, primitive
+// rebuild, pressure relaxation, energy correction) is always FP64.  Every
+// expression keeps the association order of the reference expression it
+// evaluates (built with -ffp-contract=off) so that the FP64 path reproduces the
+// reference's CPU arithmetic.
 //
 // Reference files (paths relative to the reference src/):
 //   EOS             Eos/EosSG.cpp:66-257, Eos/EosIG.cpp:64-255
@@ -53,6 +54,7 @@ struct Params {
   int relax;                // pressure relaxation active
   int bc[6];
   int lim[4];
+  int tiles[3];             // tiles per axis
   EosDev eos[4];
   const double* dcc[3];     // cell-to-cell distance of face m (cells m | m+1)
   const double* dfl[3];     // face m to its left cell centre
@@ -74,36 +76,41 @@ struct Counters {
 };
 
 // std::min / std::max tie semantics.
-__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+template <typename R> __device__ __forceinline__ R dmin(R a, R b) { return (b < a) ? b : a; }
+template <typename R> __device__ __forceinline__ R dmax(R a, R b) { return (a < b) ? b : a; }
 
 // ---------------------------------------------------------------------------
 // EOS
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double eos_energy(const EosDev& q, double eps, double rho, double p) {
-  return (q.sg ? (p + q.gpinf) : p) / q.gm1 / dmax(rho, eps) + q.eref;
+template <typename R>
+__device__ __forceinline__ R eos_energy(const EosDev& q, R eps, R rho, R p) {
+  return (q.sg ? (p + R(q.gpinf)) : p) / R(q.gm1) / dmax<R>(rho, eps) + R(q.eref);
 }
-__device__ __forceinline__ double eos_pressure(const EosDev& q, double rho, double e) {
-  double v = q.gm1 * rho * (e - q.eref);
-  return q.sg ? v - q.gpinf : v;
+template <typename R>
+__device__ __forceinline__ R eos_pressure(const EosDev& q, R rho, R e) {
+  R v = R(q.gm1) * rho * (e - R(q.eref));
+  return q.sg ? v - R(q.gpinf) : v;
 }
-__device__ __forceinline__ double eos_sound(const EosDev& q, double eps, double rho, double p) {
-  return sqrt(q.g * (q.sg ? (p + q.pinf) : p) / dmax(rho, eps));
+template <typename R>
+__device__ __forceinline__ R eos_sound(const EosDev& q, R eps, R rho, R p) {
+  return sqrt(R(q.g) * (q.sg ? (p + R(q.pinf)) : p) / dmax<R>(rho, eps));
 }
-__device__ __forceinline__ double eos_p_isentropic(const EosDev& q, double eps, double p0, double rho0, double rhof) {
-  double base = pow(rhof / dmax(rho0, eps), q.g);
-  return q.sg ? (p0 + q.pinf) * base - q.pinf : p0 * base;
+template <typename R>
+__device__ __forceinline__ R eos_p_isentropic(const EosDev& q, R eps, R p0, R rho0, R rhof) {
+  R base = pow(rhof / dmax<R>(rho0, eps), R(q.g));
+  return q.sg ? (p0 + R(q.pinf)) * base - R(q.pinf) : p0 * base;
 }
-__device__ __forceinline__ double eos_clamp_p(const EosDev& q, double p) {
-  if (q.sg) return (p <= q.pclamp) ? q.pclamp : p;
-  return (p < 1.e-8) ? 1.e-8 : p;
+template <typename R>
+__device__ __forceinline__ R eos_clamp_p(const EosDev& q, R p) {
+  if (q.sg) return (p <= R(q.pclamp)) ? R(q.pclamp) : p;
+  return (p < R(1.e-8)) ? R(1.e-8) : p;
 }
 // Tools::returnNonZeroValue (Tools.cpp:134-145)
 __device__ __forceinline__ double nonzero(double a, double eps) {
   if (a > eps || a < -eps) return a;
   return (double)((0. < a) - (a < 0.)) * eps;
 }
-// computeDensityPfinal (EosSG.cpp:163-170, EosIG.cpp:161-168)
+// computeDensityPfinal (EosSG.cpp:163-170, EosIG.cpp:161-168); FP64 only (relaxation).
 __device__ __forceinline__ double eos_rho_pfinal(const EosDev& q, double eps, double p0, double rho0, double pf,
                                                  double& drhodp) {
   double num = q.sg ? q.g * (pf + q.pinf) : q.g * pf;
@@ -117,23 +124,27 @@ __device__ __forceinline__ double eos_rho_pfinal(const EosDev& q, double eps, do
 // Limiters (Order2/LimiterMinmod.cpp:46-56, LimiterVanLeer.cpp:43-49,
 // LimiterVanAlbada.cpp:43-51, LimiterSuperBee.cpp:46-55, LimiterMC.cpp:43-53)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double limit(int kind, double s1, double s2) {
-  double prod = s1 * s2, out = 0.;
+template <typename R>
+__device__ __forceinline__ R limit(int kind, R s1, R s2) {
+  R prod = s1 * s2, out = R(0.);
   if (kind == LIM_MINMOD) {
-    if (prod > 1e-9) { out = dmin(fabs(s1), fabs(s2)); if (s1 < 0) out = -out; }
+    if (prod > R(1e-9)) { out = dmin<R>(fabs(s1), fabs(s2)); if (s1 < R(0)) out = -out; }
   } else if (kind == LIM_VANLEER) {
-    double sum = s1 + s2;
-    if ((fabs(sum) > 1e-6) && (prod > 1e-6)) out = 2. * prod / sum;
+    R sum = s1 + s2;
+    if ((fabs(sum) > R(1e-6)) && (prod > R(1e-6))) out = R(2.) * prod / sum;
   } else if (kind == LIM_VANALBADA) {
-    double sum = s1 + s2;
-    if ((fabs(s1) > 1e-6) && (fabs(s2) > 1e-6) && (fabs(sum) > 1e-6) && (prod > 1e-6))
+    R sum = s1 + s2;
+    if ((fabs(s1) > R(1e-6)) && (fabs(s2) > R(1e-6)) && (fabs(sum) > R(1e-6)) && (prod > R(1e-6)))
       out = prod * sum / (s1 * s1 + s2 * s2);
   } else if (kind == LIM_SUPERBEE) {
-    if (prod > 1e-6) out = dmax(dmin(2. * fabs(s1), fabs(s2)), dmin(fabs(s1), 2. * fabs(s2)));
-    if (s1 < 0.) out = -out;
+    if (prod > R(1e-6)) out = dmax<R>(dmin<R>(R(2.) * fabs(s1), fabs(s2)), dmin<R>(fabs(s1), R(2.) * fabs(s2)));
+    if (s1 < R(0.)) out = -out;
   } else if (kind == LIM_MC) {
-    if (prod > 1e-9) { out = dmin(2. * fabs(s1), 2. * fabs(s2)); out = dmin(out, 0.5 * (fabs(s1) + fabs(s2))); }
-    if (s1 < 0.) out = -out;
+    if (prod > R(1e-9)) {
+      out = dmin<R>(R(2.) * fabs(s1), R(2.) * fabs(s2));
+      out = dmin<R>(out, R(0.5) * (fabs(s1) + fabs(s2)));
+    }
+    if (s1 < R(0.)) out = -out;
   }
   return out;
 }
@@ -143,81 +154,83 @@ __device__ __forceinline__ double limit(int kind, double s1, double s2) {
 // ---------------------------------------------------------------------------
 // Persistent primitives plus derived quantities of one cell.  For Euler only
 // index 0 of the phase arrays is used (a[0] unused).
-template <int N>
+template <int N, typename R = double>
 struct Cell {
-  double a[N], r[N], p[N];
-  double e[N], c[N];
-  double u[3];
-  double rho, pm, em, cF;
-  double E;
+  R a[N], r[N], p[N];
+  R e[N], c[N];
+  R u[3];
+  R rho, pm, em, cF;
+  R E;
 };
 
-template <int N>
+template <int N, typename R = double>
 struct Cons {
-  double alpha[N], mass[N], energ[N];
-  double mom[3];
-  double energM;
+  R alpha[N], mass[N], energ[N];
+  R mom[3];
+  R energM;
 };
 
-template <int N>
+template <int N, typename R = double>
 struct Flux {
-  Cons<N> f;
-  double sM, uStar;
+  Cons<N, R> f;
+  R sM, uStar;
 };
 
 // Face frames: local (n, t, b) components are a signed permutation of the
 // global velocity (MeshCartesian.cpp:282, 315, 348, 387, 411, 440, 482, 511, 535).
 enum { FR_XI = 0, FR_YI, FR_ZI, FR_XM, FR_XP, FR_YM, FR_YP, FR_ZM, FR_ZP };
 template <int FR> struct Frame;
-template <> struct Frame<FR_XI> { static constexpr int i0 = 0, i1 = 1, i2 = 2; static constexpr double s0 = 1, s1 = 1, s2 = 1; };
-template <> struct Frame<FR_XP> { static constexpr int i0 = 0, i1 = 1, i2 = 2; static constexpr double s0 = 1, s1 = 1, s2 = 1; };
-template <> struct Frame<FR_XM> { static constexpr int i0 = 0, i1 = 1, i2 = 2; static constexpr double s0 = -1, s1 = -1, s2 = 1; };
-template <> struct Frame<FR_YI> { static constexpr int i0 = 1, i1 = 0, i2 = 2; static constexpr double s0 = 1, s1 = -1, s2 = 1; };
-template <> struct Frame<FR_YP> { static constexpr int i0 = 1, i1 = 0, i2 = 2; static constexpr double s0 = 1, s1 = -1, s2 = 1; };
-template <> struct Frame<FR_YM> { static constexpr int i0 = 1, i1 = 0, i2 = 2; static constexpr double s0 = -1, s1 = 1, s2 = 1; };
-template <> struct Frame<FR_ZI> { static constexpr int i0 = 2, i1 = 0, i2 = 1; static constexpr double s0 = 1, s1 = 1, s2 = 1; };
-template <> struct Frame<FR_ZP> { static constexpr int i0 = 2, i1 = 0, i2 = 1; static constexpr double s0 = 1, s1 = 1, s2 = 1; };
-template <> struct Frame<FR_ZM> { static constexpr int i0 = 2, i1 = 0, i2 = 1; static constexpr double s0 = -1, s1 = -1, s2 = 1; };
+template <> struct Frame<FR_XI> { static constexpr int i0 = 0, i1 = 1, i2 = 2; static constexpr int s0 = 1, s1 = 1, s2 = 1; };
+template <> struct Frame<FR_XP> { static constexpr int i0 = 0, i1 = 1, i2 = 2; static constexpr int s0 = 1, s1 = 1, s2 = 1; };
+template <> struct Frame<FR_XM> { static constexpr int i0 = 0, i1 = 1, i2 = 2; static constexpr int s0 = -1, s1 = -1, s2 = 1; };
+template <> struct Frame<FR_YI> { static constexpr int i0 = 1, i1 = 0, i2 = 2; static constexpr int s0 = 1, s1 = -1, s2 = 1; };
+template <> struct Frame<FR_YP> { static constexpr int i0 = 1, i1 = 0, i2 = 2; static constexpr int s0 = 1, s1 = -1, s2 = 1; };
+template <> struct Frame<FR_YM> { static constexpr int i0 = 1, i1 = 0, i2 = 2; static constexpr int s0 = -1, s1 = 1, s2 = 1; };
+template <> struct Frame<FR_ZI> { static constexpr int i0 = 2, i1 = 0, i2 = 1; static constexpr int s0 = 1, s1 = 1, s2 = 1; };
+template <> struct Frame<FR_ZP> { static constexpr int i0 = 2, i1 = 0, i2 = 1; static constexpr int s0 = 1, s1 = 1, s2 = 1; };
+template <> struct Frame<FR_ZM> { static constexpr int i0 = 2, i1 = 0, i2 = 1; static constexpr int s0 = -1, s1 = -1, s2 = 1; };
 
-template <int FR>
-__device__ __forceinline__ void to_local(const double* u, double* l) {
+template <int FR, typename R>
+__device__ __forceinline__ void to_local(const R* u, R* l) {
   using F = Frame<FR>;
   l[0] = F::s0 < 0 ? -u[F::i0] : u[F::i0];
   l[1] = F::s1 < 0 ? -u[F::i1] : u[F::i1];
   l[2] = F::s2 < 0 ? -u[F::i2] : u[F::i2];
 }
-template <int FR>
-__device__ __forceinline__ void to_global(const double* l, double* g) {
+template <int FR, typename R>
+__device__ __forceinline__ void to_global(const R* l, R* g) {
   using F = Frame<FR>;
   g[F::i0] = F::s0 < 0 ? -l[0] : l[0];
   g[F::i1] = F::s1 < 0 ? -l[1] : l[1];
   g[F::i2] = F::s2 < 0 ? -l[2] : l[2];
 }
 
-__device__ __forceinline__ double sqn3(const double* u) { return u[0] * u[0] + u[1] * u[1] + u[2] * u[2]; }
+template <typename R>
+__device__ __forceinline__ R sqn3(const R* u) { return u[0] * u[0] + u[1] * u[1] + u[2] * u[2]; }
 
 // PhaseUEq::extendedCalculusPhase + MixUEq::computeMixtureVariables; Euler:
 // PhaseEuler::extendedCalculusPhase (also rebuilds E when rebuildE).
-template <int MODEL, int N>
-__device__ __forceinline__ void derive(const Params& P, Cell<N>& s, bool rebuildE) {
+template <int MODEL, int N, typename R>
+__device__ __forceinline__ void derive(const Params& P, Cell<N, R>& s, bool rebuildE) {
+  const R eps = R(P.eps);
   if (MODEL == M_EULER) {
-    s.e[0] = eos_energy(P.eos[0], P.eps, s.r[0], s.p[0]);
-    s.c[0] = eos_sound(P.eos[0], P.eps, s.r[0], s.p[0]);
-    if (rebuildE) s.E = s.e[0] + 0.5 * sqn3(s.u);
+    s.e[0] = eos_energy<R>(P.eos[0], eps, s.r[0], s.p[0]);
+    s.c[0] = eos_sound<R>(P.eos[0], eps, s.r[0], s.p[0]);
+    if (rebuildE) s.E = s.e[0] + R(0.5) * sqn3<R>(s.u);
     return;
   }
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    s.e[k] = eos_energy(P.eos[k], P.eps, s.r[k], s.p[k]);
-    s.c[k] = eos_sound(P.eos[k], P.eps, s.r[k], s.p[k]);
+    s.e[k] = eos_energy<R>(P.eos[k], eps, s.r[k], s.p[k]);
+    s.c[k] = eos_sound<R>(P.eos[k], eps, s.r[k], s.p[k]);
   }
-  double rho = 0., pm = 0.;
+  R rho = R(0.), pm = R(0.);
 #pragma unroll
   for (int k = 0; k < N; k++) { rho += s.a[k] * s.r[k]; pm += s.a[k] * s.p[k]; }
-  double den = dmax(rho, P.eps), em = 0., cf = 0.;
+  R den = dmax<R>(rho, eps), em = R(0.), cf = R(0.);
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    double Y = s.a[k] * s.r[k] / den;
+    R Y = s.a[k] * s.r[k] / den;
     em += Y * s.e[k];
     cf += Y * s.c[k] * s.c[k];
   }
@@ -225,58 +238,58 @@ __device__ __forceinline__ void derive(const Params& P, Cell<N>& s, bool rebuild
 }
 
 // verifyAndCorrectPhase (PhaseUEq.cpp:355-370, PhaseEuler.cpp:395-399)
-template <int MODEL, int N>
-__device__ __forceinline__ void verify_correct(const Params& P, Cell<N>& s, int k) {
+template <int MODEL, int N, typename R>
+__device__ __forceinline__ void verify_correct(const Params& P, Cell<N, R>& s, int k) {
   if (MODEL == M_EULER) {
-    if (s.r[0] < 1.e-10) s.r[0] = 1.e-10;
-    s.p[0] = eos_clamp_p(P.eos[0], s.p[0]);
+    if (s.r[0] < R(1.e-10)) s.r[0] = R(1.e-10);
+    s.p[0] = eos_clamp_p<R>(P.eos[0], s.p[0]);
     return;
   }
   if (P.eps > 1.e-20) {
-    if (s.a[k] < 0.) s.a[k] = 0.;
-    if (s.a[k] > 1.) s.a[k] = 1.;
-    if (s.r[k] <= 1.e-15) s.r[k] = 1.e-15;
+    if (s.a[k] < R(0.)) s.a[k] = R(0.);
+    if (s.a[k] > R(1.)) s.a[k] = R(1.);
+    if (s.r[k] <= R(1.e-15)) s.r[k] = R(1.e-15);
   } else {
-    if (s.a[k] < 1e-15) s.a[k] = 1e-14;
-    if (s.a[k] > 1. - 1e-15) s.a[k] = 1. - 1e-14;
-    if (s.r[k] < 1.e-15) s.r[k] = 1.e-14;
+    if (s.a[k] < R(1e-15)) s.a[k] = R(1e-14);
+    if (s.a[k] > R(1.) - R(1e-15)) s.a[k] = R(1.) - R(1e-14);
+    if (s.r[k] < R(1.e-15)) s.r[k] = R(1.e-14);
   }
-  s.p[k] = eos_clamp_p(P.eos[k], s.p[k]);
+  s.p[k] = eos_clamp_p<R>(P.eos[k], s.p[k]);
 }
 
 // ---------------------------------------------------------------------------
 // Riemann solvers in the face frame; lu = local velocity of each side.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void cfl_bound(double& dtm, double s, double dx) {
-  if (fabs(s) > 1.e-3) dtm = dmin(dtm, 1. * dx / fabs(s));
+template <typename R>
+__device__ __forceinline__ void cfl_bound(R& dtm, R s, R dx) {
+  if (fabs(s) > R(1.e-3)) dtm = dmin<R>(dtm, R(1.) * dx / fabs(s));
 }
 
 // ModEuler::solveRiemannIntern
-template <int N>
-__device__ __forceinline__ void hllc_euler(const Cell<N>& Lc, const double* lu, const double EL, const Cell<N>& Rc,
-                                           const double* ru, const double ER, double lcfl, double& dtm,
-                                           Flux<N>& F) {
-  const double uL = lu[0], vL = lu[1], wL = lu[2], pL = Lc.p[0], rhoL = Lc.r[0], cL = Lc.c[0];
-  const double uR = ru[0], vR = ru[1], wR = ru[2], pR = Rc.p[0], rhoR = Rc.r[0], cR = Rc.c[0];
-  const double sL = dmin(uL - cL, uR - cR);
-  const double sR = dmax(uR + cR, uL + cL);
-  cfl_bound(dtm, sL, lcfl);
-  cfl_bound(dtm, sR, lcfl);
-  const double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
-  double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
-  if (fabs(sM) < 1.e-8) sM = 0.;
-  double mass, m0, m1, m2, en;
-  if (sL > 0.) {
+template <int N, typename R>
+__device__ __forceinline__ void hllc_euler(const Cell<N, R>& Lc, const R* lu, const R EL, const Cell<N, R>& Rc,
+                                           const R* ru, const R ER, R lcfl, R& dtm, Flux<N, R>& F) {
+  const R uL = lu[0], vL = lu[1], wL = lu[2], pL = Lc.p[0], rhoL = Lc.r[0], cL = Lc.c[0];
+  const R uR = ru[0], vR = ru[1], wR = ru[2], pR = Rc.p[0], rhoR = Rc.r[0], cR = Rc.c[0];
+  const R sL = dmin<R>(uL - cL, uR - cR);
+  const R sR = dmax<R>(uR + cR, uL + cL);
+  cfl_bound<R>(dtm, sL, lcfl);
+  cfl_bound<R>(dtm, sR, lcfl);
+  const R mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+  R sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  if (fabs(sM) < R(1.e-8)) sM = R(0.);
+  R mass, m0, m1, m2, en;
+  if (sL > R(0.)) {
     mass = rhoL * uL; m0 = rhoL * uL * uL + pL; m1 = rhoL * vL * uL; m2 = rhoL * wL * uL; en = (rhoL * EL + pL) * uL;
-  } else if (sR < 0.) {
+  } else if (sR < R(0.)) {
     mass = rhoR * uR; m0 = rhoR * uR * uR + pR; m1 = rhoR * vR * uR; m2 = rhoR * wR * uR; en = (rhoR * ER + pR) * uR;
   } else {
-    const bool left = (sM >= 0.);
-    const double mS = left ? mL : mR, uS = left ? uL : uR, pS = left ? pL : pR, sS = left ? sL : sR;
-    const double ES = left ? EL : ER, vS = left ? vL : vR, wS = left ? wL : wR;
-    const double pStar = mS * (sM - uS) + pS;
-    const double rhoStar = mS / (sS - sM);
-    const double Estar = ES + (sM - uS) * (sM + pS / mS);
+    const bool left = (sM >= R(0.));
+    const R mS = left ? mL : mR, uS = left ? uL : uR, pS = left ? pL : pR, sS = left ? sL : sR;
+    const R ES = left ? EL : ER, vS = left ? vL : vR, wS = left ? wL : wR;
+    const R pStar = mS * (sM - uS) + pS;
+    const R rhoStar = mS / (sS - sM);
+    const R Estar = ES + (sM - uS) * (sM + pS / mS);
     mass = rhoStar * sM; m0 = rhoStar * sM * sM + pStar; m1 = rhoStar * sM * vS; m2 = rhoStar * sM * wS;
     en = (rhoStar * Estar + pStar) * sM;
   }
@@ -285,24 +298,25 @@ __device__ __forceinline__ void hllc_euler(const Cell<N>& Lc, const double* lu, 
 }
 
 // ModUEq::solveRiemannIntern
-template <int N>
-__device__ __forceinline__ void hllc_ueq(const Params& P, const Cell<N>& Lc, const double* lu, const Cell<N>& Rc,
-                                         const double* ru, double& dtm, Flux<N>& F) {
-  const double uL = lu[0], cL = Lc.cF, pL = Lc.pm, rhoL = Lc.rho;
-  const double uR = ru[0], cR = Rc.cF, pR = Rc.pm, rhoR = Rc.rho;
-  const double sL = dmin(uL - cL, uR - cR);
-  const double sR = dmax(uR + cR, uL + cL);
-  cfl_bound(dtm, sL, P.lcfl);
-  cfl_bound(dtm, sR, P.lcfl);
-  const double mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
-  double sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
-  if (fabs(sM) < 1.e-8) sM = 0.;
-  if (sL >= 0. || sR <= 0.) {
-    const bool left = (sL >= 0.);
-    const Cell<N>& S = left ? Lc : Rc;
-    const double* su = left ? lu : ru;
-    const double uS = su[0], rhoS = left ? rhoL : rhoR, pS = left ? pL : pR;
-    const double totalEnergy = S.em + 0.5 * sqn3(su);
+template <int N, typename R>
+__device__ __forceinline__ void hllc_ueq(const Params& P, const Cell<N, R>& Lc, const R* lu, const Cell<N, R>& Rc,
+                                         const R* ru, R& dtm, Flux<N, R>& F) {
+  const R eps = R(P.eps), lcfl = R(P.lcfl);
+  const R uL = lu[0], cL = Lc.cF, pL = Lc.pm, rhoL = Lc.rho;
+  const R uR = ru[0], cR = Rc.cF, pR = Rc.pm, rhoR = Rc.rho;
+  const R sL = dmin<R>(uL - cL, uR - cR);
+  const R sR = dmax<R>(uR + cR, uL + cL);
+  cfl_bound<R>(dtm, sL, lcfl);
+  cfl_bound<R>(dtm, sR, lcfl);
+  const R mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
+  R sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  if (fabs(sM) < R(1.e-8)) sM = R(0.);
+  if (sL >= R(0.) || sR <= R(0.)) {
+    const bool left = (sL >= R(0.));
+    const Cell<N, R>& S = left ? Lc : Rc;
+    const R* su = left ? lu : ru;
+    const R uS = su[0], rhoS = left ? rhoL : rhoR, pS = left ? pL : pR;
+    const R totalEnergy = S.em + R(0.5) * sqn3<R>(su);
 #pragma unroll
     for (int k = 0; k < N; k++) {
       F.f.alpha[k] = S.a[k] * sM;
@@ -315,20 +329,20 @@ __device__ __forceinline__ void hllc_ueq(const Params& P, const Cell<N>& Lc, con
     F.f.energM = (rhoS * totalEnergy + pS) * uS;
     F.uStar = uS;
   } else {
-    const bool left = (sM >= 0.);
-    const Cell<N>& S = left ? Lc : Rc;
-    const double* su = left ? lu : ru;
-    const double uS = su[0], pS = left ? pL : pR, mS = left ? mL : mR, sS = left ? sL : sR;
-    const double totalEnergy = S.em + 0.5 * sqn3(su);
-    const double rhoStar = mS / (sS - sM);
-    const double EStar = totalEnergy + (sM - uS) * (sM + pS / mS);
-    const double pStar = mS * (sM - uS) + pS;
+    const bool left = (sM >= R(0.));
+    const Cell<N, R>& S = left ? Lc : Rc;
+    const R* su = left ? lu : ru;
+    const R uS = su[0], pS = left ? pL : pR, mS = left ? mL : mR, sS = left ? sL : sR;
+    const R totalEnergy = S.em + R(0.5) * sqn3<R>(su);
+    const R rhoStar = mS / (sS - sM);
+    const R EStar = totalEnergy + (sM - uS) * (sM + pS / mS);
+    const R pStar = mS * (sM - uS) + pS;
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      const double mk = S.r[k] * (sS - uS);
-      const double rks = mk / (sS - sM);
-      const double pks = eos_p_isentropic(P.eos[k], P.eps, S.p[k], S.r[k], rks);
-      const double eks = eos_energy(P.eos[k], P.eps, rks, pks);
+      const R mk = S.r[k] * (sS - uS);
+      const R rks = mk / (sS - sM);
+      const R pks = eos_p_isentropic<R>(P.eos[k], eps, S.p[k], S.r[k], rks);
+      const R eks = eos_energy<R>(P.eos[k], eps, rks, pks);
       F.f.alpha[k] = S.a[k] * sM;
       F.f.mass[k] = S.a[k] * rks * sM;
       F.f.energ[k] = S.a[k] * rks * eks * sM;
@@ -344,127 +358,159 @@ __device__ __forceinline__ void hllc_ueq(const Params& P, const Cell<N>& Lc, con
 
 // Face flux between two derived cells (global-frame velocities), returned in
 // the global frame.  EL/ER: total energies the Euler solver uses.
-template <int MODEL, int N, int FR>
-__device__ __forceinline__ void face_flux(const Params& P, const Cell<N>& L, double EL, const Cell<N>& R, double ER,
-                                          double& dtm, Flux<N>& F) {
-  double lu[3], ru[3], loc[3];
-  to_local<FR>(L.u, lu);
-  to_local<FR>(R.u, ru);
-  if (MODEL == M_EULER) hllc_euler<N>(L, lu, EL, R, ru, ER, P.lcfl, dtm, F);
-  else hllc_ueq<N>(P, L, lu, R, ru, dtm, F);
+template <int MODEL, int N, int FR, typename R>
+__device__ __forceinline__ void face_flux(const Params& P, const Cell<N, R>& L, R EL, const Cell<N, R>& Rr, R ER,
+                                          R& dtm, Flux<N, R>& F) {
+  R lu[3], ru[3], loc[3];
+  to_local<FR, R>(L.u, lu);
+  to_local<FR, R>(Rr.u, ru);
+  if (MODEL == M_EULER) hllc_euler<N, R>(L, lu, EL, Rr, ru, ER, R(P.lcfl), dtm, F);
+  else hllc_ueq<N, R>(P, L, lu, Rr, ru, dtm, F);
   loc[0] = F.f.mom[0]; loc[1] = F.f.mom[1]; loc[2] = F.f.mom[2];
-  to_global<FR>(loc, F.f.mom);
+  to_global<FR, R>(loc, F.f.mom);
 }
 
 // Boundary face (BoundCond::solveRiemann, BoundCond.cpp:78-98): the buffer cell
 // is the interior cell with its derived state rebuilt (Euler: E rebuilt).
-template <int MODEL, int N, int FR>
-__device__ __forceinline__ void boundary_flux(const Params& P, const Cell<N>& B, double EB, int bc, double& dtm,
-                                              Flux<N>& F) {
+template <int MODEL, int N, int FR, typename R>
+__device__ __forceinline__ void boundary_flux(const Params& P, const Cell<N, R>& B, R EB, int bc, R& dtm,
+                                              Flux<N, R>& F) {
   if (bc == BC_WALL) {
-    double lu[3];
-    to_local<FR>(B.u, lu);
-    const double uL = lu[0];
-    const double cL = (MODEL == M_EULER) ? B.c[0] : B.cF;
-    const double pL = (MODEL == M_EULER) ? B.p[0] : B.pm;
-    const double rhoL = (MODEL == M_EULER) ? B.r[0] : B.rho;
-    const double sL = dmin(uL - cL, -uL - cL);
-    cfl_bound(dtm, sL, P.lcfl);
-    const double pStar = (MODEL == M_EULER) ? rhoL * uL * (uL - sL) + pL : rhoL * (uL - sL) * uL + pL;
+    R lu[3];
+    to_local<FR, R>(B.u, lu);
+    const R uL = lu[0];
+    const R cL = (MODEL == M_EULER) ? B.c[0] : B.cF;
+    const R pL = (MODEL == M_EULER) ? B.p[0] : B.pm;
+    const R rhoL = (MODEL == M_EULER) ? B.r[0] : B.rho;
+    const R sL = dmin<R>(uL - cL, -uL - cL);
+    cfl_bound<R>(dtm, sL, R(P.lcfl));
+    const R pStar = (MODEL == M_EULER) ? rhoL * uL * (uL - sL) + pL : rhoL * (uL - sL) * uL + pL;
 #pragma unroll
-    for (int k = 0; k < N; k++) { F.f.alpha[k] = 0.; F.f.mass[k] = 0.; F.f.energ[k] = 0.; }
-    double loc[3] = {pStar, 0., 0.};
-    to_global<FR>(loc, F.f.mom);
-    F.f.energM = 0.;
-    F.sM = 0.;
-    F.uStar = 0.;
+    for (int k = 0; k < N; k++) { F.f.alpha[k] = R(0.); F.f.mass[k] = R(0.); F.f.energ[k] = R(0.); }
+    R loc[3] = {pStar, R(0.), R(0.)};
+    to_global<FR, R>(loc, F.f.mom);
+    F.f.energM = R(0.);
+    F.sM = R(0.);
+    F.uStar = R(0.);
   } else {
-    face_flux<MODEL, N, FR>(P, B, EB, B, EB, dtm, F);
+    face_flux<MODEL, N, FR, R>(P, B, EB, B, EB, dtm, F);
   }
 }
+
+// ---------------------------------------------------------------------------
+// Accumulation of face contributions into the cell's m_cons buffer.
+// FP64: plain double sums in the reference order.  FP32: float-float pairs
+// (TwoSum of every contribution) so the sum keeps ~48 significant bits.
+// ---------------------------------------------------------------------------
+template <typename R> struct Acc;
+template <> struct Acc<double> {
+  double v = 0.;
+  __device__ __forceinline__ void add(double x) { v += x; }
+  __device__ __forceinline__ void sub(double x) { v -= x; }
+  __device__ __forceinline__ double value() const { return v; }
+};
+template <> struct Acc<float> {
+  float hi = 0.f, lo = 0.f;
+  __device__ __forceinline__ void add(float x) {
+    const float s = hi + x;
+    const float bb = s - hi;
+    const float err = (hi - (s - bb)) + (x - bb);
+    hi = s;
+    lo += err;
+  }
+  __device__ __forceinline__ void sub(float x) { add(-x); }
+  __device__ __forceinline__ double value() const { return (double)hi + (double)lo; }
+};
+
+template <int N, typename R>
+struct ConsAcc {
+  Acc<R> alpha[N], mass[N], energ[N];
+  Acc<R> mom[3];
+  Acc<R> energM;
+};
 
 // CellInterface::addFlux / subtractFlux with the non-conservative terms of the
 // cell's time-n phases (FluxUEq.cpp:61-84, 224-244; FluxPUEq.cpp:44-65).
-template <int MODEL, int N>
-__device__ __forceinline__ void add_flux(Cons<N>& A, const Flux<N>& F, double coefA, const double* an,
-                                         const double* pn) {
+template <int MODEL, int N, int D, typename R>
+__device__ __forceinline__ void add_flux(ConsAcc<N, R>& A, const Flux<N, R>& F, R coefA, const R* an, const R* pn) {
   if (MODEL == M_EULER) {
-    A.mass[0] += coefA * F.f.mass[0];
+    A.mass[0].add(coefA * F.f.mass[0]);
   } else {
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      A.alpha[k] += coefA * F.f.alpha[k];
-      A.mass[k] += coefA * F.f.mass[k];
-      A.energ[k] += coefA * F.f.energ[k];
+      A.alpha[k].add(coefA * F.f.alpha[k]);
+      A.mass[k].add(coefA * F.f.mass[k]);
+      A.energ[k].add(coefA * F.f.energ[k]);
     }
   }
-  A.mom[0] += F.f.mom[0] * coefA;
-  A.mom[1] += F.f.mom[1] * coefA;
-  A.mom[2] += F.f.mom[2] * coefA;
-  A.energM += coefA * F.f.energM;
+#pragma unroll
+  for (int d = 0; d < D; d++) A.mom[d].add(F.f.mom[d] * coefA);
+  A.energM.add(coefA * F.f.energM);
   if (MODEL != M_EULER) {
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      A.alpha[k] += -coefA * an[k] * F.sM;
-      if (MODEL == M_PUEQ) A.energ[k] += coefA * an[k] * pn[k] * F.uStar;
+      A.alpha[k].add(-coefA * an[k] * F.sM);
+      if (MODEL == M_PUEQ) A.energ[k].add(coefA * an[k] * pn[k] * F.uStar);
     }
   }
 }
 
-template <int MODEL, int N>
-__device__ __forceinline__ void sub_flux(Cons<N>& A, const Flux<N>& F, double coefA, const double* an,
-                                         const double* pn) {
+template <int MODEL, int N, int D, typename R>
+__device__ __forceinline__ void sub_flux(ConsAcc<N, R>& A, const Flux<N, R>& F, R coefA, const R* an, const R* pn) {
   if (MODEL == M_EULER) {
-    A.mass[0] -= coefA * F.f.mass[0];
+    A.mass[0].sub(coefA * F.f.mass[0]);
   } else {
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      A.alpha[k] -= coefA * F.f.alpha[k];
-      A.mass[k] -= coefA * F.f.mass[k];
-      A.energ[k] -= coefA * F.f.energ[k];
+      A.alpha[k].sub(coefA * F.f.alpha[k]);
+      A.mass[k].sub(coefA * F.f.mass[k]);
+      A.energ[k].sub(coefA * F.f.energ[k]);
     }
   }
-  A.mom[0] -= F.f.mom[0] * coefA;
-  A.mom[1] -= F.f.mom[1] * coefA;
-  A.mom[2] -= F.f.mom[2] * coefA;
-  A.energM -= coefA * F.f.energM;
+#pragma unroll
+  for (int d = 0; d < D; d++) A.mom[d].sub(F.f.mom[d] * coefA);
+  A.energM.sub(coefA * F.f.energM);
   if (MODEL != M_EULER) {
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      A.alpha[k] -= -coefA * an[k] * F.sM;
-      if (MODEL == M_PUEQ) A.energ[k] -= coefA * an[k] * pn[k] * F.uStar;
+      A.alpha[k].sub(-coefA * an[k] * F.sM);
+      if (MODEL == M_PUEQ) A.energ[k].sub(coefA * an[k] * pn[k] * F.uStar);
     }
   }
 }
 
-// Cell::timeEvolution (Cell.cpp:286-299): A <- scale*A (+ scheme correction) + U^n.
-template <int MODEL, int N>
-__device__ __forceinline__ void time_evolution(const Params& P, Cons<N>& A, const Cell<N>& n, double scale) {
+// Cell::timeEvolution (Cell.cpp:286-299): A <- scale*sum (+ scheme correction) + U^n, FP64.
+template <int MODEL, int N, int D, typename R>
+__device__ __forceinline__ void time_evolution(const Params& P, const ConsAcc<N, R>& S, Cons<N>& A,
+                                               const Cell<N>& n, double scale) {
   Cons<N> U;
   if (MODEL == M_EULER) {
     U.mass[0] = n.r[0];
     U.mom[0] = n.u[0] * U.mass[0]; U.mom[1] = n.u[1] * U.mass[0]; U.mom[2] = n.u[2] * U.mass[0];
     U.energM = U.mass[0] * n.E;
-    A.mass[0] *= scale;
+    A.mass[0] = S.mass[0].value() * scale;
   } else {
 #pragma unroll
     for (int k = 0; k < N; k++) {
       U.alpha[k] = n.a[k];
       U.mass[k] = n.a[k] * n.r[k];
-      U.energ[k] = n.a[k] * n.r[k] * eos_energy(P.eos[k], P.eps, n.r[k], n.p[k]);
-      A.alpha[k] *= scale; A.mass[k] *= scale; A.energ[k] *= scale;
+      U.energ[k] = n.a[k] * n.r[k] * eos_energy<double>(P.eos[k], P.eps, n.r[k], n.p[k]);
+      A.alpha[k] = S.alpha[k].value() * scale;
+      A.mass[k] = S.mass[k].value() * scale;
+      A.energ[k] = S.energ[k].value() * scale;
     }
     U.mom[0] = n.u[0] * n.rho; U.mom[1] = n.u[1] * n.rho; U.mom[2] = n.u[2] * n.rho;
     U.energM = n.rho * n.E;
   }
-  A.mom[0] *= scale; A.mom[1] *= scale; A.mom[2] *= scale;
-  A.energM *= scale;
+#pragma unroll
+  for (int d = 0; d < 3; d++) A.mom[d] = (d < D) ? S.mom[d].value() * scale : 0.;
+  A.energM = S.energM.value() * scale;
   if (MODEL == M_UEQ) {
     double rhoNp1 = 0., sumD = 0., m[3];
 #pragma unroll
     for (int k = 0; k < N; k++) { rhoNp1 += A.mass[k] + U.mass[k]; sumD += A.energ[k]; }
     m[0] = A.mom[0] + U.mom[0]; m[1] = A.mom[1] + U.mom[1]; m[2] = A.mom[2] + U.mom[2];
-    const double dRU2 = sqn3(m) / rhoNp1 - sqn3(U.mom) / n.rho;
+    const double dRU2 = sqn3<double>(m) / rhoNp1 - sqn3<double>(U.mom) / n.rho;
     const double err = (A.energM - 0.5 * dRU2) - sumD;
 #pragma unroll
     for (int k = 0; k < N; k++) A.energ[k] += n.a[k] * err;
@@ -481,7 +527,7 @@ __device__ __forceinline__ void time_evolution(const Params& P, Cons<N>& A, cons
   A.energM += 1. * U.energM;
 }
 
-// Flux::buildPrim (FluxUEq.cpp:161-207, FluxEuler.cpp:124-150); derived rebuilt.
+// Flux::buildPrim (FluxUEq.cpp:161-207, FluxEuler.cpp:124-150); derived rebuilt.  FP64.
 template <int MODEL, int N>
 __device__ __forceinline__ void build_prim(const Params& P, Cons<N>& A, Cell<N>& s) {
   if (MODEL == M_EULER) {
@@ -492,10 +538,10 @@ __device__ __forceinline__ void build_prim(const Params& P, Cons<N>& A, Cell<N>&
     if (fabs(s.u[1]) < 1.e-8) s.u[1] = 0.;
     if (fabs(s.u[2]) < 1.e-8) s.u[2] = 0.;
     s.E = A.energM / m;
-    const double e = s.E - 0.5 * sqn3(s.u);
+    const double e = s.E - 0.5 * sqn3<double>(s.u);
     s.e[0] = e;
-    s.p[0] = eos_pressure(P.eos[0], m, e);
-    s.c[0] = eos_sound(P.eos[0], P.eps, m, s.p[0]);
+    s.p[0] = eos_pressure<double>(P.eos[0], m, e);
+    s.c[0] = eos_sound<double>(P.eos[0], P.eps, m, s.p[0]);
     return;
   }
   double un = 0., rhoMel = 0.;
@@ -522,20 +568,20 @@ __device__ __forceinline__ void build_prim(const Params& P, Cons<N>& A, Cell<N>&
     s.a[k] = A.alpha[k];
     s.r[k] = A.mass[k] / dmax(A.alpha[k], P.eps);
     const double ek = A.energ[k] / dmax(A.mass[k], P.eps);
-    s.p[k] = eos_pressure(P.eos[k], s.r[k], ek);
-    verify_correct<MODEL, N>(P, s, k);
+    s.p[k] = eos_pressure<double>(P.eos[k], s.r[k], ek);
+    verify_correct<MODEL, N, double>(P, s, k);
   }
   s.u[0] = A.mom[0] / rhoMel; s.u[1] = A.mom[1] / rhoMel; s.u[2] = A.mom[2] / rhoMel;
   if (fabs(s.u[0]) < 1.e-8) s.u[0] = 0.;
   if (fabs(s.u[1]) < 1.e-8) s.u[1] = 0.;
   if (fabs(s.u[2]) < 1.e-8) s.u[2] = 0.;
-  derive<MODEL, N>(P, s, false);
+  derive<MODEL, N, double>(P, s, false);
   s.E = A.energM / rhoMel;
 }
 
-// RelaxationPInfinite::relaxation + RelaxationP::NewtonRaphson from initial
-// guess p0 (the reference's guess is sum alpha_k p_k).  Returns iterations
-// (0 when gated out).
+// RelaxationPInfinite::relaxation + RelaxationP::NewtonRaphson from the initial
+// guess pStar (the reference's guess is sum alpha_k p_k).  Returns iterations
+// (0 when gated out).  FP64.
 template <int N>
 __device__ __forceinline__ int relax_newton(const Params& P, Cell<N>& s, double pStar) {
   if (P.eps > 1.e-20) {
@@ -552,7 +598,7 @@ __device__ __forceinline__ int relax_newton(const Params& P, Cell<N>& s, double 
     it++;
     pStar -= f / df;
 #pragma unroll
-    for (int k = 0; k < N; k++) pStar = eos_clamp_p(P.eos[k], pStar);
+    for (int k = 0; k < N; k++) pStar = eos_clamp_p<double>(P.eos[k], pStar);
     f = -1.;
     df = 0.;
 #pragma unroll
@@ -584,7 +630,7 @@ __device__ __forceinline__ int relax_newton(const Params& P, Cell<N>& s, double 
     }
     pI /= sumZk * (double)(N - 1);
 #pragma unroll
-    for (int k = 0; k < N; k++) pI = eos_clamp_p(P.eos[k], pI);
+    for (int k = 0; k < N; k++) pI = eos_clamp_p<double>(P.eos[k], pI);
 #pragma unroll
     for (int k = 0; k < N; k++) {
       double drhodp;
@@ -605,11 +651,11 @@ __device__ __forceinline__ double relax_guess(const Cell<N>& s) {
   return p;
 }
 
-// Cell::correctionEnergy + FluxPUEq::correctionEnergy (Kapila only), then fulfillState.
+// Cell::correctionEnergy + FluxPUEq::correctionEnergy (Kapila only), then fulfillState.  FP64.
 template <int MODEL, int N>
 __device__ __forceinline__ void correct_and_derive(const Params& P, Cell<N>& s) {
   if (MODEL == M_PUEQ) {
-    const double em = s.E - 0.5 * sqn3(s.u);
+    const double em = s.E - 0.5 * sqn3<double>(s.u);
     double pp = s.rho * em, denom = 0.;
 #pragma unroll
     for (int k = 0; k < N; k++) {
@@ -620,10 +666,10 @@ __device__ __forceinline__ void correct_and_derive(const Params& P, Cell<N>& s) 
 #pragma unroll
     for (int k = 0; k < N; k++) {
       s.p[k] = pp;
-      verify_correct<MODEL, N>(P, s, k);
+      verify_correct<MODEL, N, double>(P, s, k);
     }
   }
-  derive<MODEL, N>(P, s, MODEL == M_EULER);
+  derive<MODEL, N, double>(P, s, MODEL == M_EULER);
 }
 
 }  // namespace ecg

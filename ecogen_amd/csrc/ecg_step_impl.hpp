@@ -1,118 +1,138 @@
-// Kernel templates of the finite-volume step.  One thread owns one cell.
+// Kernel templates of the finite-volume step: face-centric tiles.
 //
-// First order (Run::solveHyperbolic, Run.cpp:696-716, then solveRelaxations,
-// Run.cpp:771-804) is a single kernel: the cell gathers its 2*D face fluxes in
-// face-creation order (X faces, Y faces, Z faces, then boundary faces X-, X+,
-// Y-, Y+, Z-, Z+; MeshCartesian.cpp:283-557), updates, relaxes, corrects and
-// writes the new persistent state.
+// A 256-thread block owns a tile of cells (256 x 1, 16 x 16 or 16 x 4 x 4) and
+// walks the axes in the reference's face-creation order (X, Y, Z; then the
+// boundary faces X-, X+, Y-, Y+, Z-, Z+ of MeshCartesian.cpp:283-557).  For
+// each axis it
+//   A. (second order) evaluates every face slope the tile needs once into LDS
+//      (CellInterfaceO2Cartesian::computeSlopes, CellInterfaceO2Cartesian.cpp:77-99),
+//   B. evaluates every interior face flux of the tile once into LDS: MUSCL
+//      states (CellO2Cartesian::computeLocalSlopes + extrapolation,
+//      CellO2Cartesian.cpp:23-171, CellInterfaceO2Cartesian.cpp:101-202) and the
+//      Riemann solver,
+//   C. lets each cell add its lower face and subtract its upper face
+//      (CellInterface::addFlux / subtractFlux, CellInterface.cpp:267-297).
+// Then each thread adds its boundary faces and performs the FP64 cell update,
+// primitive rebuild, relaxation and energy correction (Run.cpp:696-716, 771-804;
+// CellO2::predictionOrdre2, CellO2.cpp:111-135) and writes the new state.
 //
-// Second order (Run::solveHyperbolicO2, Run.cpp:627-692) is two launches of the
-// same kernel: the predictor evaluates MUSCL fluxes on W^n and writes the
-// half-step state W* (CellO2::predictionOrdre2, CellO2.cpp:111-135); the
-// corrector evaluates them on W* and updates W^n to W^{n+1}.
+// Second order is two launches: the predictor evaluates fluxes on W^n and writes
+// W*; the corrector evaluates them on W* and updates W^n to W^{n+1}.
 #pragma once
 
 #include "ecg_kernels.hpp"
 
 namespace ecg {
 
+template <int D> struct Tile;
+template <> struct Tile<1> { static constexpr int X = 256, Y = 1, Z = 1; };
+template <> struct Tile<2> { static constexpr int X = 16, Y = 16, Z = 1; };
+template <> struct Tile<3> { static constexpr int X = 16, Y = 4, Z = 4; };
+
+template <int D, int A> struct TileAxis {
+  static constexpr int T = (A == 0) ? Tile<D>::X : (A == 1) ? Tile<D>::Y : Tile<D>::Z;
+  static constexpr int LINES = 256 / T;
+  static constexpr int NSLOPE = (T + 3) * LINES;
+  static constexpr int NFACE = (T + 1) * LINES;
+};
+
+// Variables kept per face slope: phase (alpha, rho, p) x N (Euler: rho, p), velocity on the active axes.
+template <int MODEL, int N, int D> struct Sizes {
+  static constexpr int PV = (MODEL == M_EULER) ? 2 : 3 * N;
+  static constexpr int NS = PV + D;
+  // flux: alpha, mass, energ per phase (Euler: mass), momentum (D), energy, sM, uStar
+  static constexpr int NF = ((MODEL == M_EULER) ? 1 : 3 * N) + D + 1 + 2;
+  template <int A> static constexpr int lds_axis(bool o2) {
+    return (o2 ? TileAxis<D, A>::NSLOPE * NS : 0) + TileAxis<D, A>::NFACE * NF;
+  }
+  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+  static constexpr int lds_elems(bool o2) {
+    return cmax(lds_axis<0>(o2), cmax(D >= 2 ? lds_axis<1>(o2) : 0, D >= 3 ? lds_axis<2>(o2) : 0));
+  }
+};
+
 // Persistent variable layout (see include/ecogen_amd.h).
-template <int MODEL, int N>
-struct Layout {
-  static constexpr int phase_vars = (MODEL == M_EULER) ? 2 : 3 * N;
-  template <int D> static constexpr int nvar() { return phase_vars + D + 1; }
-};
-
-template <int MODEL, int N, int D>
-__device__ __forceinline__ void load_prim(const Params& P, const double* __restrict__ W, int64_t c, Cell<N>& s) {
+template <int MODEL, int N, int D, typename S, typename R>
+__device__ __forceinline__ void load_prim(const Params& P, const S* __restrict__ W, int64_t c, Cell<N, R>& s) {
   const int64_t ld = P.pitch;
   int v = 0;
   if (MODEL == M_EULER) {
-    s.a[0] = 1.;
-    s.r[0] = W[(v++) * ld + c];
-    s.p[0] = W[(v++) * ld + c];
+    s.a[0] = R(1.);
+    s.r[0] = R(W[(v++) * ld + c]);
+    s.p[0] = R(W[(v++) * ld + c]);
   } else {
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      s.a[k] = W[(v++) * ld + c];
-      s.r[k] = W[(v++) * ld + c];
-      s.p[k] = W[(v++) * ld + c];
+      s.a[k] = R(W[(v++) * ld + c]);
+      s.r[k] = R(W[(v++) * ld + c]);
+      s.p[k] = R(W[(v++) * ld + c]);
     }
   }
-  s.u[0] = W[(v++) * ld + c];
-  s.u[1] = (D >= 2) ? W[(v++) * ld + c] : 0.;
-  s.u[2] = (D >= 3) ? W[(v++) * ld + c] : 0.;
-  s.E = W[v * ld + c];
+  s.u[0] = R(W[(v++) * ld + c]);
+  s.u[1] = (D >= 2) ? R(W[(v++) * ld + c]) : R(0.);
+  s.u[2] = (D >= 3) ? R(W[(v++) * ld + c]) : R(0.);
+  s.E = R(W[v * ld + c]);
 }
 
-template <int MODEL, int N, int D>
-__device__ __forceinline__ void store_prim(const Params& P, double* __restrict__ W, int64_t c, const Cell<N>& s) {
+template <int MODEL, int N, int D, typename S>
+__device__ __forceinline__ void store_prim(const Params& P, S* __restrict__ W, int64_t c, const Cell<N>& s) {
   const int64_t ld = P.pitch;
   int v = 0;
   if (MODEL == M_EULER) {
-    W[(v++) * ld + c] = s.r[0];
-    W[(v++) * ld + c] = s.p[0];
+    W[(v++) * ld + c] = S(s.r[0]);
+    W[(v++) * ld + c] = S(s.p[0]);
   } else {
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      W[(v++) * ld + c] = s.a[k];
-      W[(v++) * ld + c] = s.r[k];
-      W[(v++) * ld + c] = s.p[k];
+      W[(v++) * ld + c] = S(s.a[k]);
+      W[(v++) * ld + c] = S(s.r[k]);
+      W[(v++) * ld + c] = S(s.p[k]);
     }
   }
-  W[(v++) * ld + c] = s.u[0];
-  if (D >= 2) W[(v++) * ld + c] = s.u[1];
-  if (D >= 3) W[(v++) * ld + c] = s.u[2];
-  W[v * ld + c] = s.E;
+  W[(v++) * ld + c] = S(s.u[0]);
+  if (D >= 2) W[(v++) * ld + c] = S(s.u[1]);
+  if (D >= 3) W[(v++) * ld + c] = S(s.u[2]);
+  W[v * ld + c] = S(s.E);
 }
 
-// alpha of phase 0 at time n (PhaseEuler::getAlpha returns 0, PhaseEuler.h:110)
-template <int MODEL>
-__device__ __forceinline__ double alpha0(const Params& P, const double* __restrict__ W, int64_t c) {
-  return (MODEL == M_EULER) ? 0. : W[c];
-}
+template <typename R, typename S>
+__device__ __forceinline__ R alpha0_at(const S* __restrict__ W, int64_t c) { return R(W[c]); }
 
-// Interior face slope variables: phase (alpha, rho, p) per phase, velocity (3).
-template <int N>
-struct Slope {
-  double a[N], r[N], p[N];
-  double u[3];
-};
-
-template <int MODEL, int N, int D>
-__device__ __forceinline__ void face_slope(const Cell<N>& L, const Cell<N>& R, double dist, Slope<N>& s) {
-  // CellInterfaceO2Cartesian::computeSlopes (CellInterfaceO2Cartesian.cpp:77-99)
+// Slope of face (L | L+1): (W_R - W_L) / distance, phase variables then velocity.
+template <int MODEL, int N, int D, typename R>
+__device__ __forceinline__ void slope_vars(const Cell<N, R>& L, const Cell<N, R>& Rr, R dist, R* s) {
+  int v = 0;
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    if (MODEL != M_EULER) s.a[k] = (R.a[k] - L.a[k]) / dist;
-    s.r[k] = (R.r[k] - L.r[k]) / dist;
-    s.p[k] = (R.p[k] - L.p[k]) / dist;
+    if (MODEL != M_EULER) s[v++] = (Rr.a[k] - L.a[k]) / dist;
+    s[v++] = (Rr.r[k] - L.r[k]) / dist;
+    s[v++] = (Rr.p[k] - L.p[k]) / dist;
   }
 #pragma unroll
-  for (int d = 0; d < 3; d++) s.u[d] = (d < D) ? (R.u[d] - L.u[d]) / dist : 0.;
+  for (int d = 0; d < D; d++) s[v++] = (Rr.u[d] - L.u[d]) / dist;
 }
 
-// CellO2Cartesian::computeLocalSlopes (CellO2Cartesian.cpp:23-171) on a uniform
-// Cartesian cell: slope1 is the face's own slope, slope2 the opposite face's
-// slope when that face is interior (boundary faces carry none: slope2 = 0 and
-// the opposite alpha stays 0); then limiting and extrapolation by dist with the
-// sign sgn (+1 left cell, -1 right cell: changeSign, CellInterfaceO2Cartesian.cpp:151-156).
-template <int MODEL, int N, int D>
-__device__ __forceinline__ void extrapolate(const Params& P, Cell<N>& b, const Slope<N>& s1, const Slope<N>& s2,
-                                            bool has2, double aAfter, double aCell, double aOther, double dist,
-                                            bool negate) {
+// computeLocalSlopes + limitSlopes + extrapolate + verifyAndCorrectPhase on a
+// uniform Cartesian cell: slope1 is the face's own slope, slope2 the opposite
+// face's slope when that face is interior (boundary faces carry none: slope2
+// and the opposite alpha stay 0).  negate: right cell (changeSign,
+// CellInterfaceO2Cartesian.cpp:151-156).
+template <int MODEL, int N, int D, typename R>
+__device__ __forceinline__ void extrapolate(const Params& P, Cell<N, R>& b, const R* s1, const R* s2, bool has2,
+                                            R aAfter, R aCell, R aOther, R dist, bool negate) {
   int lg, lvf;
-  if ((aCell >= 1.e-4) && (aCell <= 1. - 1.e-4) && ((aOther - aCell) * (aCell - aAfter) >= 1.e-8)) {
+  if ((aCell >= R(1.e-4)) && (aCell <= R(1.) - R(1.e-4)) && ((aOther - aCell) * (aCell - aAfter) >= R(1.e-8))) {
     lg = P.lim[1]; lvf = P.lim[3];
   } else {
     lg = P.lim[0]; lvf = P.lim[2];
   }
+  int v = 0;
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    double la = 0., lr, lp;
-    if (MODEL != M_EULER) la = limit(lvf, s1.a[k], has2 ? s2.a[k] : 0.);
-    lr = limit(lg, s1.r[k], has2 ? s2.r[k] : 0.);
-    lp = limit(lg, s1.p[k], has2 ? s2.p[k] : 0.);
+    R la = R(0.), lr, lp;
+    if (MODEL != M_EULER) { la = limit<R>(lvf, s1[v], has2 ? s2[v] : R(0.)); v++; }
+    lr = limit<R>(lg, s1[v], has2 ? s2[v] : R(0.)); v++;
+    lp = limit<R>(lg, s1[v], has2 ? s2[v] : R(0.)); v++;
     if (negate) { la = -la; lr = -lr; lp = -lp; }
     if (MODEL != M_EULER) b.a[k] += la * dist;
     b.r[k] += lr * dist;
@@ -120,112 +140,172 @@ __device__ __forceinline__ void extrapolate(const Params& P, Cell<N>& b, const S
     if (MODEL == M_EULER) {
 #pragma unroll
       for (int d = 0; d < D; d++) {
-        double lu = limit(lg, s1.u[d], has2 ? s2.u[d] : 0.);
-        if (negate) lu = lu * -1.;
+        R lu = limit<R>(lg, s1[v + d], has2 ? s2[v + d] : R(0.));
+        if (negate) lu = lu * R(-1.);
         b.u[d] = b.u[d] + lu * dist;
       }
     }
-    verify_correct<MODEL, N>(P, b, k);
+    verify_correct<MODEL, N, R>(P, b, k);
   }
   if (MODEL != M_EULER) {
 #pragma unroll
     for (int d = 0; d < D; d++) {
-      double lu = limit(lg, s1.u[d], has2 ? s2.u[d] : 0.);
-      if (negate) lu = lu * -1.;
+      R lu = limit<R>(lg, s1[v + d], has2 ? s2[v + d] : R(0.));
+      if (negate) lu = lu * R(-1.);
       b.u[d] = b.u[d] + lu * dist;
     }
   }
 }
 
-// Extrapolated left or right state of the interior face (x | x+1) along an axis
-// (CellInterfaceO2Cartesian::solveRiemann, CellInterfaceO2Cartesian.cpp:101-202).
-// m: axis index of cell x; n: cells along the axis; st: index stride.
-template <int MODEL, int N, int D, bool RIGHT>
-__device__ __forceinline__ void muscl_state(const Params& P, const double* __restrict__ wsrc,
-                                            const double* __restrict__ wn, int64_t x, int m, int n, int64_t st,
-                                            int axis, Cell<N>& b) {
-  Cell<N> cL, cR, cO;
-  load_prim<MODEL, N, D>(P, wsrc, x, cL);
-  load_prim<MODEL, N, D>(P, wsrc, x + st, cR);
-  Slope<N> s1, s2;
-  face_slope<MODEL, N, D>(cL, cR, P.dcc[axis][m], s1);
-  const double aL = (MODEL == M_EULER) ? 0. : cL.a[0];
-  const double aR = (MODEL == M_EULER) ? 0. : cR.a[0];
-  if (!RIGHT) {
-    const bool has2 = (m >= 1);
-    double aLL = 0.;
-    if (has2) {
-      load_prim<MODEL, N, D>(P, wsrc, x - st, cO);
-      face_slope<MODEL, N, D>(cO, cL, P.dcc[axis][m - 1], s2);
-      aLL = alpha0<MODEL>(P, wn, x - st);
-    }
-    b = cL;
-    extrapolate<MODEL, N, D>(P, b, s1, s2, has2, aLL, aL, aR, P.dfl[axis][m], false);
+template <int MODEL, int N, int D, typename R>
+__device__ __forceinline__ void flux_to_lds(const Flux<N, R>& F, R* o) {
+  int v = 0;
+  if (MODEL == M_EULER) {
+    o[v++] = F.f.mass[0];
   } else {
-    const bool has2 = (m + 1 <= n - 2);
-    double aRR = 0.;
-    if (has2) {
-      load_prim<MODEL, N, D>(P, wsrc, x + 2 * st, cO);
-      face_slope<MODEL, N, D>(cR, cO, P.dcc[axis][m + 1], s2);
-      aRR = alpha0<MODEL>(P, wn, x + 2 * st);
-    }
-    b = cR;
-    extrapolate<MODEL, N, D>(P, b, s1, s2, has2, aRR, aR, aL, P.dfr[axis][m], true);
+#pragma unroll
+    for (int k = 0; k < N; k++) { o[v++] = F.f.alpha[k]; o[v++] = F.f.mass[k]; o[v++] = F.f.energ[k]; }
   }
-  derive<MODEL, N>(P, b, MODEL == M_EULER);
+#pragma unroll
+  for (int d = 0; d < D; d++) o[v++] = F.f.mom[d];
+  o[v++] = F.f.energM;
+  o[v++] = F.sM;
+  o[v++] = F.uStar;
 }
 
-template <int MODEL, int N, int D, bool O2, int FR>
-__device__ __forceinline__ void interior_pair(const Params& P, const StepLaunch& L, const Cell<N>& own, int64_t c,
-                                              int m, int n, int64_t st, int axis, double coef, const double* an,
-                                              const double* pn, double& dtm, Cons<N>& acc) {
-  Flux<N> F;
-  if (!O2) {
-    // First order: the stored cells (derived quantities rebuilt identically).
-    if (m > 0) {
-      Cell<N> nb;
-      load_prim<MODEL, N, D>(P, L.wsrc, c - st, nb);
-      derive<MODEL, N>(P, nb, false);
-      face_flux<MODEL, N, FR>(P, nb, nb.E, own, own.E, dtm, F);
-      add_flux<MODEL, N>(acc, F, coef, an, pn);
-    }
-    if (m < n - 1) {
-      Cell<N> nb;
-      load_prim<MODEL, N, D>(P, L.wsrc, c + st, nb);
-      derive<MODEL, N>(P, nb, false);
-      face_flux<MODEL, N, FR>(P, own, own.E, nb, nb.E, dtm, F);
-      sub_flux<MODEL, N>(acc, F, coef, an, pn);
-    }
+template <int MODEL, int N, int D, typename R>
+__device__ __forceinline__ void flux_from_lds(const R* o, Flux<N, R>& F) {
+  int v = 0;
+  if (MODEL == M_EULER) {
+    F.f.mass[0] = o[v++];
   } else {
-    if (m > 0) {
-      Cell<N> bl, br;
-      muscl_state<MODEL, N, D, false>(P, L.wsrc, L.wn, c - st, m - 1, n, st, axis, bl);
-      muscl_state<MODEL, N, D, true>(P, L.wsrc, L.wn, c - st, m - 1, n, st, axis, br);
-      face_flux<MODEL, N, FR>(P, bl, bl.E, br, br.E, dtm, F);
-      add_flux<MODEL, N>(acc, F, coef, an, pn);
-    }
-    if (m < n - 1) {
-      Cell<N> bl, br;
-      muscl_state<MODEL, N, D, false>(P, L.wsrc, L.wn, c, m, n, st, axis, bl);
-      muscl_state<MODEL, N, D, true>(P, L.wsrc, L.wn, c, m, n, st, axis, br);
-      face_flux<MODEL, N, FR>(P, bl, bl.E, br, br.E, dtm, F);
-      sub_flux<MODEL, N>(acc, F, coef, an, pn);
-    }
+#pragma unroll
+    for (int k = 0; k < N; k++) { F.f.alpha[k] = o[v++]; F.f.mass[k] = o[v++]; F.f.energ[k] = o[v++]; }
   }
+#pragma unroll
+  for (int d = 0; d < D; d++) F.f.mom[d] = o[v++];
+  F.f.energM = o[v++];
+  F.sM = o[v++];
+  F.uStar = o[v++];
 }
 
-template <int MODEL, int N, int FR>
-__device__ __forceinline__ void boundary_face(const Params& P, const Cell<N>& b, int bc, double coef,
-                                              const double* an, const double* pn, double& dtm, Cons<N>& acc) {
-  Flux<N> F;
-  const double EB = (MODEL == M_EULER) ? b.e[0] + 0.5 * sqn3(b.u) : b.E;
-  boundary_flux<MODEL, N, FR>(P, b, EB, bc, dtm, F);
-  sub_flux<MODEL, N>(acc, F, coef, an, pn);
+// Tile origin of the block, XCD-aware: when the tile count divides by the 8
+// XCDs, each XCD receives a contiguous range of tiles (neighbours share an L2).
+template <int D>
+__device__ __forceinline__ void tile_origin(const Params& P, int& i0, int& j0, int& k0) {
+  const int nb = P.tiles[0] * P.tiles[1] * P.tiles[2];
+  int b = blockIdx.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const int tx = b % P.tiles[0];
+  const int rest = b / P.tiles[0];
+  const int ty = rest % P.tiles[1];
+  const int tz = rest / P.tiles[1];
+  i0 = tx * Tile<D>::X;
+  j0 = ty * Tile<D>::Y;
+  k0 = tz * Tile<D>::Z;
+}
+
+// One axis pass (steps A, B, C above) for axis AX with face frame FR.
+template <int MODEL, int N, int D, bool O2, typename R, int AX, int FR>
+__device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>& L, R* lds, const int org[3],
+                                          const int own[3], bool own_ok, const R* an, const R* pn, R& dtm,
+                                          ConsAcc<N, R>& acc) {
+  using TA = TileAxis<D, AX>;
+  using SZ = Sizes<MODEL, N, D>;
+  constexpr int T = TA::T;
+  constexpr int B = (AX == 0) ? 1 : 0;           // first line axis
+  constexpr int C = (AX == 2) ? 1 : 2;           // second line axis
+  constexpr int TB = (B == 0) ? Tile<D>::X : Tile<D>::Y;
+  const int n3[3] = {P.nx, P.ny, P.nz};
+  const int64_t st3[3] = {1, (int64_t)P.nx, (int64_t)P.nx * P.ny};
+  const int na = n3[AX];
+  const int64_t sa = st3[AX];
+  R* s_slope = lds;
+  R* s_flux = lds + (O2 ? TA::NSLOPE * SZ::NS : 0);
+  const int tid = threadIdx.x;
+
+  // A. face slopes s_g, g = -2 .. T (interior faces only)
+  if (O2) {
+    for (int q = tid; q < TA::NSLOPE; q += 256) {
+      const int g = q % (T + 3) - 2;
+      const int l = q / (T + 3);
+      const int cb = org[B] + l % TB, cc = org[C] + l / TB;
+      const int m = org[AX] + g;
+      R* o = s_slope + q * SZ::NS;
+      if (cb < n3[B] && cc < n3[C] && m >= 0 && m <= na - 2) {
+        const int64_t cell = (int64_t)m * sa + (int64_t)cb * st3[B] + (int64_t)cc * st3[C];
+        Cell<N, R> cl, cr;
+        load_prim<MODEL, N, D>(P, L.wsrc, cell, cl);
+        load_prim<MODEL, N, D>(P, L.wsrc, cell + sa, cr);
+        slope_vars<MODEL, N, D, R>(cl, cr, R(P.dcc[AX][m]), o);
+      }
+    }
+    __syncthreads();
+  }
+
+  // B. interior face fluxes f = -1 .. T-1
+  for (int q = tid; q < TA::NFACE; q += 256) {
+    const int f = q % (T + 1) - 1;
+    const int l = q / (T + 1);
+    const int cb = org[B] + l % TB, cc = org[C] + l / TB;
+    const int m = org[AX] + f;
+    if (cb < n3[B] && cc < n3[C] && m >= 0 && m <= na - 2) {
+      const int64_t cell = (int64_t)m * sa + (int64_t)cb * st3[B] + (int64_t)cc * st3[C];
+      Cell<N, R> bl, br;
+      load_prim<MODEL, N, D>(P, L.wsrc, cell, bl);
+      load_prim<MODEL, N, D>(P, L.wsrc, cell + sa, br);
+      if (O2) {
+        const R aL = (MODEL == M_EULER) ? R(0.) : bl.a[0];
+        const R aR = (MODEL == M_EULER) ? R(0.) : br.a[0];
+        const R* s1 = s_slope + (l * (T + 3) + f + 2) * SZ::NS;
+        const bool hasm = (m >= 1), hasp = (m + 1 <= na - 2);
+        const R aLL = (MODEL != M_EULER && hasm) ? alpha0_at<R>(L.wn, cell - sa) : R(0.);
+        const R aRR = (MODEL != M_EULER && hasp) ? alpha0_at<R>(L.wn, cell + 2 * sa) : R(0.);
+        extrapolate<MODEL, N, D, R>(P, bl, s1, s1 - SZ::NS, hasm, aLL, aL, aR, R(P.dfl[AX][m]), false);
+        extrapolate<MODEL, N, D, R>(P, br, s1, s1 + SZ::NS, hasp, aRR, aR, aL, R(P.dfr[AX][m]), true);
+        derive<MODEL, N, R>(P, bl, MODEL == M_EULER);
+        derive<MODEL, N, R>(P, br, MODEL == M_EULER);
+      } else {
+        derive<MODEL, N, R>(P, bl, false);
+        derive<MODEL, N, R>(P, br, false);
+      }
+      Flux<N, R> F;
+      face_flux<MODEL, N, FR, R>(P, bl, bl.E, br, br.E, dtm, F);
+      flux_to_lds<MODEL, N, D, R>(F, s_flux + q * SZ::NF);
+    }
+  }
+  __syncthreads();
+
+  // C. gather: add the lower face, subtract the upper face
+  if (own_ok) {
+    const int t = own[AX] - org[AX];
+    const int l = (own[B] - org[B]) + TB * (own[C] - org[C]);
+    const R coef = R(P.coef[AX]);
+    Flux<N, R> F;
+    if (own[AX] > 0) {
+      flux_from_lds<MODEL, N, D, R>(s_flux + (l * (T + 1) + t) * SZ::NF, F);
+      add_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
+    }
+    if (own[AX] < na - 1) {
+      flux_from_lds<MODEL, N, D, R>(s_flux + (l * (T + 1) + t + 1) * SZ::NF, F);
+      sub_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
+    }
+  }
+  __syncthreads();
+}
+
+template <int MODEL, int N, int D, int FR, typename R>
+__device__ __forceinline__ void boundary_face(const Params& P, const Cell<N, R>& b, int bc, R coef, const R* an,
+                                              const R* pn, R& dtm, ConsAcc<N, R>& acc) {
+  Flux<N, R> F;
+  const R EB = (MODEL == M_EULER) ? b.e[0] + R(0.5) * sqn3<R>(b.u) : b.E;
+  boundary_flux<MODEL, N, FR, R>(P, b, EB, bc, dtm, F);
+  sub_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
 }
 
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = dmin(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o > 0; o >>= 1) v = dmin<double>(v, __shfl_xor(v, o, 64));
   return v;
 }
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
@@ -234,83 +314,106 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
-template <int MODEL, int N, int D, bool O2>
-__global__ __launch_bounds__(256) void k_step(Params P, StepLaunch L, Clock* __restrict__ clk,
+template <int MODEL, int N, int D, bool O2, typename R>
+__global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock* __restrict__ clk,
                                               Counters* __restrict__ cnt) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  R* lds = reinterpret_cast<R*>(lds_raw);
   __shared__ double s_dt[4];
   __shared__ unsigned long long s_c[3][4];
   if (clk->done) return;
   const double dt = clk->dt;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double dtm = 1.e10;
-  unsigned long long n_it = 0, n_fail = 0, n_relax = 0;
-  if (c < P.ncell) {
-    const int i = (int)(c % P.nx);
-    const int j = (D >= 2) ? (int)((c / P.nx) % P.ny) : 0;
-    const int k = (D >= 3) ? (int)(c / ((int64_t)P.nx * P.ny)) : 0;
-    // Time-n cell: non-conservative terms and U^n (Cell::getPhase default type).
-    Cell<N> own;
-    load_prim<MODEL, N, D>(P, L.wn, c, own);
-    derive<MODEL, N>(P, own, false);
-    // Cell the fluxes see: W^n (first order / predictor) or W* (corrector).
-    Cell<N> src;
-    if (L.stage == 2) {
-      load_prim<MODEL, N, D>(P, L.wsrc, c, src);
-      derive<MODEL, N>(P, src, false);
-    } else {
-      src = own;
-    }
-    double an[N], pn[N];
-#pragma unroll
-    for (int q = 0; q < N; q++) { an[q] = own.a[q]; pn[q] = own.p[q]; }
-    Cons<N> acc;
-#pragma unroll
-    for (int q = 0; q < N; q++) { acc.alpha[q] = 0.; acc.mass[q] = 0.; acc.energ[q] = 0.; }
-    acc.mom[0] = 0.; acc.mom[1] = 0.; acc.mom[2] = 0.; acc.energM = 0.;
+  int org[3];
+  tile_origin<D>(P, org[0], org[1], org[2]);
+  const int tid = threadIdx.x;
+  int own[3];
+  own[0] = org[0] + tid % Tile<D>::X;
+  own[1] = org[1] + (tid / Tile<D>::X) % Tile<D>::Y;
+  own[2] = org[2] + tid / (Tile<D>::X * Tile<D>::Y);
+  const bool own_ok = own[0] < P.nx && own[1] < P.ny && own[2] < P.nz;
+  const int64_t c = own[0] + (int64_t)own[1] * P.nx + (int64_t)own[2] * P.nx * P.ny;
 
-    interior_pair<MODEL, N, D, O2, FR_XI>(P, L, src, c, i, P.nx, 1, 0, P.coef[0], an, pn, dtm, acc);
-    if (D >= 2) interior_pair<MODEL, N, D, O2, FR_YI>(P, L, src, c, j, P.ny, P.nx, 1, P.coef[1], an, pn, dtm, acc);
-    if (D >= 3)
-      interior_pair<MODEL, N, D, O2, FR_ZI>(P, L, src, c, k, P.nz, (int64_t)P.nx * P.ny, 2, P.coef[2], an, pn, dtm, acc);
-    if (i == 0) boundary_face<MODEL, N, FR_XM>(P, src, P.bc[0], P.coef[0], an, pn, dtm, acc);
-    if (i == P.nx - 1) boundary_face<MODEL, N, FR_XP>(P, src, P.bc[1], P.coef[0], an, pn, dtm, acc);
+  R dtm = R(1.e10);
+  unsigned long long n_it = 0, n_fail = 0, n_relax = 0;
+  // Time-n cell: non-conservative terms and U^n (Cell::getPhase default type).
+  Cell<N, R> ownR;
+  R an[N], pn[N];
+#pragma unroll
+  for (int q = 0; q < N; q++) { an[q] = R(0.); pn[q] = R(0.); }
+  if (own_ok) {
+    load_prim<MODEL, N, D>(P, L.wn, c, ownR);
+#pragma unroll
+    for (int q = 0; q < N; q++) { an[q] = ownR.a[q]; pn[q] = ownR.p[q]; }
+  }
+  ConsAcc<N, R> acc;
+
+  axis_pass<MODEL, N, D, O2, R, 0, FR_XI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
+  if (D >= 2) axis_pass<MODEL, N, D, O2, R, 1, FR_YI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
+  if (D >= 3) axis_pass<MODEL, N, D, O2, R, 2, FR_ZI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
+
+  if (own_ok) {
+    // Boundary faces see the stage's own cell: W^n (first order / predictor) or W* (corrector).
+    Cell<N, R> src;
+    if (O2 && L.stage == 2) load_prim<MODEL, N, D>(P, L.wsrc, c, src);
+    else src = ownR;
+    derive<MODEL, N, R>(P, src, false);
+    const int i = own[0], j = own[1], k = own[2];
+    if (i == 0) boundary_face<MODEL, N, D, FR_XM, R>(P, src, P.bc[0], R(P.coef[0]), an, pn, dtm, acc);
+    if (i == P.nx - 1) boundary_face<MODEL, N, D, FR_XP, R>(P, src, P.bc[1], R(P.coef[0]), an, pn, dtm, acc);
     if (D >= 2) {
-      if (j == 0) boundary_face<MODEL, N, FR_YM>(P, src, P.bc[2], P.coef[1], an, pn, dtm, acc);
-      if (j == P.ny - 1) boundary_face<MODEL, N, FR_YP>(P, src, P.bc[3], P.coef[1], an, pn, dtm, acc);
+      if (j == 0) boundary_face<MODEL, N, D, FR_YM, R>(P, src, P.bc[2], R(P.coef[1]), an, pn, dtm, acc);
+      if (j == P.ny - 1) boundary_face<MODEL, N, D, FR_YP, R>(P, src, P.bc[3], R(P.coef[1]), an, pn, dtm, acc);
     }
     if (D >= 3) {
-      if (k == 0) boundary_face<MODEL, N, FR_ZM>(P, src, P.bc[4], P.coef[2], an, pn, dtm, acc);
-      if (k == P.nz - 1) boundary_face<MODEL, N, FR_ZP>(P, src, P.bc[5], P.coef[2], an, pn, dtm, acc);
+      if (k == 0) boundary_face<MODEL, N, D, FR_ZM, R>(P, src, P.bc[4], R(P.coef[2]), an, pn, dtm, acc);
+      if (k == P.nz - 1) boundary_face<MODEL, N, D, FR_ZP, R>(P, src, P.bc[5], R(P.coef[2]), an, pn, dtm, acc);
     }
 
-    const double scale = (L.stage == 1) ? 0.5 * dt : dt;
-    time_evolution<MODEL, N>(P, acc, own, scale);
+    // FP64 cell update from the time-n primitives.
+    Cell<N> own64;
+    load_prim<MODEL, N, D>(P, L.wn, c, own64);
+    if (MODEL != M_EULER && sizeof(R) == 4) {
+      // FP32 storage rounds a dominant volume fraction (1 - eps) by up to 3e-8
+      // absolute, so the stored fractions no longer sum to one.  buildPrim's
+      // renormalisation would turn that defect into a phase-density change,
+      // which a stiff EOS amplifies to p_inf * 1e-8 of pressure and the
+      // relaxation then pushes into the minority phase.  Restoring the sum in
+      // FP64 here keeps the stored phase densities and pressures.
+      double un = 0.;
+#pragma unroll
+      for (int k = 0; k < N; k++) un += own64.a[k];
+#pragma unroll
+      for (int k = 0; k < N; k++) own64.a[k] /= un;
+    }
+    derive<MODEL, N, double>(P, own64, false);
+    const double scale = (O2 && L.stage == 1) ? 0.5 * dt : dt;
+    Cons<N> A;
+    time_evolution<MODEL, N, D, R>(P, acc, A, own64, scale);
     Cell<N> nw;
-    build_prim<MODEL, N>(P, acc, nw);
+    build_prim<MODEL, N>(P, A, nw);
     // Relaxation: after the predictor (CellO2.cpp:128-134) and after the full step
     // (Run::advancingProcedure, Run.cpp:593).
     if (MODEL != M_EULER && P.relax) {
       const int it = relax_newton<N>(P, nw, relax_guess<N>(nw));
       if (it > 0) { n_relax = 1; n_it = (unsigned long long)it; n_fail = (it >= 100) ? 1 : 0; }
       correct_and_derive<MODEL, N>(P, nw);
-    } else if (L.stage == 1) {
+    } else if (O2 && L.stage == 1) {
       correct_and_derive<MODEL, N>(P, nw);
     }
     store_prim<MODEL, N, D>(P, L.wout, c, nw);
   }
   // Block reductions: CFL minimum and relaxation counters.
-  dtm = wave_min(dtm);
+  const double dtm64 = wave_min((double)dtm);
   n_it = wave_sum(n_it);
   n_fail = wave_sum(n_fail);
   n_relax = wave_sum(n_relax);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) { s_dt[wv] = dtm; s_c[0][wv] = n_it; s_c[1][wv] = n_fail; s_c[2][wv] = n_relax; }
+  if (lane == 0) { s_dt[wv] = dtm64; s_c[0][wv] = n_it; s_c[1][wv] = n_fail; s_c[2][wv] = n_relax; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int nw = blockDim.x >> 6;
     double m = s_dt[0];
     unsigned long long a = s_c[0][0], b = s_c[1][0], r = s_c[2][0];
-    for (int w = 1; w < nw; w++) { m = dmin(m, s_dt[w]); a += s_c[0][w]; b += s_c[1][w]; r += s_c[2][w]; }
+    for (int w = 1; w < 4; w++) { m = dmin<double>(m, s_dt[w]); a += s_c[0][w]; b += s_c[1][w]; r += s_c[2][w]; }
     atomicMin(&clk->dtmax_bits, (unsigned long long)__double_as_longlong(m));
     if (r) {
       atomicAdd(&cnt->newton_iters, a);
@@ -322,9 +425,9 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunch L, Clock* __r
 
 // Initial condition: Cell::fill over the domains (later ones override), then
 // Cell::completeFulfillState (Cell.cpp:344-353) and E = e + |u|^2/2.
-template <int MODEL, int N, int D>
+template <int MODEL, int N, int D, typename S>
 __global__ __launch_bounds__(256) void k_fill(Params P, const DomainDev* __restrict__ doms, int ndom,
-                                              double* __restrict__ W) {
+                                              S* __restrict__ W) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P.ncell) return;
   const int i = (int)(c % P.nx);
@@ -377,22 +480,34 @@ __global__ __launch_bounds__(256) void k_fill(Params P, const DomainDev* __restr
       break;
     }
   }
-  derive<MODEL, N>(P, s, MODEL == M_EULER);
-  if (MODEL != M_EULER) s.E = s.em + 0.5 * sqn3(s.u);
+  derive<MODEL, N, double>(P, s, MODEL == M_EULER);
+  if (MODEL != M_EULER) s.E = s.em + 0.5 * sqn3<double>(s.u);
   store_prim<MODEL, N, D>(P, W, c, s);
 }
 
-template <int MODEL, int N, int D>
-void launch_step_t(const Params& P, const StepLaunch& L, Clock* clk, Counters* cnt, hipStream_t s, int blocks,
-                   int threads) {
-  if (L.stage == 0) hipLaunchKernelGGL((k_step<MODEL, N, D, false>), dim3(blocks), dim3(threads), 0, s, P, L, clk, cnt);
-  else hipLaunchKernelGGL((k_step<MODEL, N, D, true>), dim3(blocks), dim3(threads), 0, s, P, L, clk, cnt);
+template <int MODEL, int N, int D, typename R>
+void launch_step_t(const Params& P, const StepLaunch& Lv, Clock* clk, Counters* cnt, hipStream_t s) {
+  StepLaunchT<R> L;
+  L.wn = (const R*)Lv.wn; L.wsrc = (const R*)Lv.wsrc; L.wout = (R*)Lv.wout; L.stage = Lv.stage;
+  const int blocks = P.tiles[0] * P.tiles[1] * P.tiles[2];
+  using SZ = Sizes<MODEL, N, D>;
+  if (Lv.stage == 0) {
+    const size_t lds = sizeof(R) * SZ::lds_elems(false);
+    auto k = k_step<MODEL, N, D, false, R>;
+    hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
+  } else {
+    const size_t lds = sizeof(R) * SZ::lds_elems(true);
+    auto k = k_step<MODEL, N, D, true, R>;
+    hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
+  }
 }
 
-template <int MODEL, int N, int D>
-void launch_fill_t(const Params& P, const DomainDev* doms, int ndom, double* w, hipStream_t s, int blocks,
-                   int threads) {
-  hipLaunchKernelGGL((k_fill<MODEL, N, D>), dim3(blocks), dim3(threads), 0, s, P, doms, ndom, w);
+template <int MODEL, int N, int D, typename S>
+void launch_fill_t(const Params& P, const DomainDev* doms, int ndom, void* w, hipStream_t s) {
+  const int blocks = (int)((P.ncell + 255) / 256);
+  hipLaunchKernelGGL((k_fill<MODEL, N, D, S>), dim3(blocks), dim3(256), 0, s, P, doms, ndom, (S*)w);
 }
 
 }  // namespace ecg

@@ -1,0 +1,26 @@
+// Instantiations of the step kernels: ueq model, float face arithmetic.
+#include "ecg_step_impl.hpp"
+
+namespace ecg {
+
+step_fn get_step_fn_ueq_f32(int nph, int dim) {
+  if (nph == 2 && dim == 1) return &launch_step_t<M_UEQ, 2, 1, float>;
+  if (nph == 2 && dim == 2) return &launch_step_t<M_UEQ, 2, 2, float>;
+  if (nph == 2 && dim == 3) return &launch_step_t<M_UEQ, 2, 3, float>;
+  if (nph == 3 && dim == 1) return &launch_step_t<M_UEQ, 3, 1, float>;
+  if (nph == 3 && dim == 2) return &launch_step_t<M_UEQ, 3, 2, float>;
+  if (nph == 3 && dim == 3) return &launch_step_t<M_UEQ, 3, 3, float>;
+  return nullptr;
+}
+
+fill_fn get_fill_fn_ueq_f32(int nph, int dim) {
+  if (nph == 2 && dim == 1) return &launch_fill_t<M_UEQ, 2, 1, float>;
+  if (nph == 2 && dim == 2) return &launch_fill_t<M_UEQ, 2, 2, float>;
+  if (nph == 2 && dim == 3) return &launch_fill_t<M_UEQ, 2, 3, float>;
+  if (nph == 3 && dim == 1) return &launch_fill_t<M_UEQ, 3, 1, float>;
+  if (nph == 3 && dim == 2) return &launch_fill_t<M_UEQ, 3, 2, float>;
+  if (nph == 3 && dim == 3) return &launch_fill_t<M_UEQ, 3, 3, float>;
+  return nullptr;
+}
+
+}  // namespace ecg

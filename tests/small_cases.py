@@ -104,8 +104,25 @@ def wall_variant(case, bc=("wall", "nonreflecting", "wall", "wall", "nonreflecti
     return c
 
 
+def pueq_3d_tiles(order=2):
+    """Several tiles along every axis (inter-tile faces and halo slopes)."""
+    c = pueq3_3d(order=order, n=10)
+    c["nx"], c["ny"], c["nz"] = 36, 9, 10
+    c["name"] = f"pueq3_3d_tiles_o{order}"
+    return c
+
+
+def pueq_2d_xcd(order=1):
+    """Tile count divisible by 8 (XCD-contiguous tile order)."""
+    c = pueq_2d(order=order, n=64)
+    c["ny"] = 32
+    c["ly"] = 0.5
+    c["name"] = f"pueq_2d_xcd_o{order}"
+    return c
+
+
 def all_cases():
-    cs = [euler_sod(1), euler_sod(2), euler_2d(1), euler_2d(2), euler_2d(2, lim="superbee"), euler_3d(1),
+    cs = [pueq_3d_tiles(2), pueq_2d_xcd(1), pueq_2d_xcd(2), euler_sod(1), euler_sod(2), euler_2d(1), euler_2d(2), euler_2d(2, lim="superbee"), euler_3d(1),
           euler_3d(2), wall_variant(euler_2d(1)), wall_variant(euler_3d(1)),
           pueq_1d(1), pueq_1d(2), pueq_2d(1), pueq_2d(2), pueq_2d(1, alpha_null=True), wall_variant(pueq_2d(1)),
           pueq3_3d(1), pueq3_3d(2),
