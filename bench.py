@@ -1,21 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the explicit finite-volume step on one MI355X.
 
-Workload: the 3D two-phase pressure-non-equilibrium case (VELOCITYEQ,
-stiffened-gas water + ideal-gas air, 512^3 cells, 4096 air pockets with radii
-log-uniform over 2-10 cells, high-pressure slab drive, MUSCL minmod with the
-explicit midpoint predictor-corrector, infinite pressure relaxation), plain
-FP64.  A step is one full reference time step of the whole mesh (predictor,
-corrector, relaxations, CFL reduction) with the state resident in HBM.
+Workload (default): the 3D two-phase pressure-non-equilibrium case
+(VELOCITYEQ, stiffened-gas water + ideal-gas air, 512^3 cells, 4096 air
+pockets with radii log-uniform over 2-10 cells, high-pressure slab drive,
+MUSCL minmod with the explicit midpoint predictor-corrector, infinite pressure
+relaxation).  Three forms run in the same process on the same case:
 
-Prints one JSON line: Mcell-updates/s over the timed steps, the HBM roofline of
-the dominant kernel (algorithmic bytes per launch over its event-timed average
-duration) and the oracle timed on the host on a bounded replica.
+  fp64          plain FP64 (the parity path)
+  fp32          FP32 state, reconstruction and fluxes with compensated sums;
+                FP64 cell update, EOS inversions and relaxation Newton
+  fp32_learned  fp32 with the Newton warm start of a 12-64-64-1 GELU network
+                (BF16 MFMA inference fused into the relaxation), trained in
+                this process on relaxation samples of plain FP64 runs
+
+A step is one full reference time step of the whole mesh (predictor,
+corrector, relaxations, CFL reduction) with the state resident in HBM.  The
+FP32 forms' per-field relative L1/L-infinity errors against the FP64 path are
+measured on a common dt sequence after the same number of steps.
+
+Prints one JSON line: value = Mcell-updates/s of the fp32_learned form; per
+form: throughput, errors, Newton work, and the HBM roofline of its dominant
+kernel (algorithmic bytes per launch over the event-timed average duration);
+the oracle timed on the host on a bounded replica.
 """
 import argparse
 import ctypes as C
 import json
-import math
 import os
 import sys
 import time
@@ -43,14 +54,14 @@ def hip():
     return L
 
 
-def traffic_from_profiles(kernel):
+def traffic_from_profiles(key):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE), if committed."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == kernel:
+        if len(parts) >= 2 and parts[0] == key:
             try:
                 return float(parts[1])
             except ValueError:
@@ -69,7 +80,8 @@ WORKLOADS = {
 }
 FORMS = {"fp64": "plain FP64",
          "fp32": "FP32 state, reconstruction and fluxes (compensated sums); FP64 cell update, EOS inversions "
-                 "and relaxation Newton"}
+                 "and relaxation Newton",
+         "fp32_learned": "fp32 + learned Newton warm start (BF16 MFMA MLP fused into the relaxation)"}
 DEFAULT_N = {"c5": 512, "c4": 256, "c3": 2048, "c2": 2048}
 
 
@@ -103,70 +115,33 @@ def cpu_baseline(workload="c5", steps=12):
                        f"{el:.1f} s on one host core")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c5")
-    ap.add_argument("--n", type=int, default=0, help="cells per axis (default: the workload's size)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--calibrate", action="store_true", help="also launch the byte-counter calibration copy")
-    ap.add_argument("--form", choices=("fp64", "fp32"), default="fp64",
-                    help="fp64: plain FP64; fp32: FP32 storage/fluxes with FP64 update and relaxation")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition is "
-              f"not part of this build", file=sys.stderr)
-        sys.exit(2)
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    from ecogen_amd import Solver
-    from ecogen_amd.case import variable_names
 
-    n = args.n or DEFAULT_N[args.workload]
-    case, npk = make_case(args.workload, n)
-    case["precision"] = args.form
-    esize = 8 if args.form == "fp64" else 4
-    print(f"[bench] building {case['name']} ({args.workload}, {npk} inclusions)", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    s = Solver(case)
-    s.synchronize()
-    print(f"[bench] initial condition ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    nvar = len(variable_names(case))
-    ncell = s.ncell
-
+def time_form(H, s, steps, warmup, esize, nvar, order):
+    """Warm up, then time `steps` device-clocked steps with HIP events on the solver's stream."""
     s.reset_clock()
-    s.enqueue(args.warmup)
-    s.synchronize()
-    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
-
-    H = hip()
+    s.enqueue(warmup)
+    st0 = s.synchronize()
     e0, e1 = C.c_void_p(), C.c_void_p()
     H.hipEventCreate(C.byref(e0))
     H.hipEventCreate(C.byref(e1))
     s.set_profiling(True)
     stream = C.c_void_p(s.stream)
-    w0 = time.perf_counter()
     H.hipEventRecord(e0, stream)
-    s.enqueue(args.steps)
+    s.enqueue(steps)
     H.hipEventRecord(e1, stream)
     H.hipEventSynchronize(e1)
-    wall = time.perf_counter() - w0
     ms = C.c_float()
     H.hipEventElapsedTime(C.byref(ms), e0, e1)
     st = s.synchronize()
     kt = s.kernel_times()
-    if args.calibrate:
-        nb = esize * nvar * ncell // 16 * 16
-        s.calibration_copy(nb)
-        print(f"[bench] calibration copy: {nb} bytes read and {nb} bytes written in one launch", file=sys.stderr,
-              flush=True)
-    elapsed = ms.value / 1e3
-    ms_per_step = ms.value / args.steps
-    value = ncell * args.steps / elapsed / 1e6
-    print(f"[bench] {args.steps} steps: {ms.value:.1f} ms (wall {wall * 1e3:.1f} ms), iteration {st['iteration']}, "
-          f"newton iters {st['newton_iters']}, fallbacks {st['newton_fail']}", file=sys.stderr, flush=True)
-
+    s.set_profiling(False)
+    H.hipEventDestroy(e0)
+    H.hipEventDestroy(e1)
+    ncell = s.ncell
+    value = ncell * steps / (ms.value / 1e3) / 1e6
     # Dominant kernel and its algorithmic bytes per launch: each stage reads and writes the
     # persistent SoA set once (predictor: W^n -> W*; corrector: W*, W^n -> W^{n+1}).
     transfers = {"predict": 2, "correct": 3, "step": 2}
@@ -175,45 +150,151 @@ def main():
     avg_s = kms / kn / 1e3
     bytes_launch = transfers[kind] * nvar * esize * ncell
     achieved = bytes_launch / avg_s / 1e9
-    kernel_name = {"predict": "k_step (second order, predictor)", "correct": "k_step (second order, corrector)",
-                   "step": "k_step (first order)"}[kind]
-    traffic = traffic_from_profiles(f"{args.workload}:{args.form}:{kind}")
+    d = {k: st[k] - st0[k] for k in ("newton_iters", "relax_calls", "warm_calls", "warm_fallbacks", "newton_fail")}
+    d["resid_sum"] = st["resid_sum"] - st0["resid_sum"]
+    return dict(value=value, ms_per_step=ms.value / steps, kind=kind, avg_launch_s=avg_s,
+                bytes_launch=bytes_launch, achieved=achieved,
+                kernel_ms={k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items() if v[1]},
+                whole_step_gbs=(5 if order == 2 else 2) * nvar * esize * value * 1e6 / 1e9, newton=d)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c5")
+    ap.add_argument("--n", type=int, default=0, help="cells per axis (default: the workload's size)")
+    ap.add_argument("--forms", default="fp64,fp32,fp32_learned")
+    ap.add_argument("--samples", type=int, default=10_000_000, help="training records for the warm start")
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the common-dt error measurement")
+    ap.add_argument("--calibrate", action="store_true", help="also launch the byte-counter calibration copy")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition is "
+              f"not part of this build", file=sys.stderr)
+        sys.exit(2)
+    forms = [f for f in args.forms.split(",") if f]
+    for f in forms:
+        if f not in FORMS:
+            sys.exit(f"unknown form {f}")
+
+    from ecogen_amd import Solver
+    from ecogen_amd.case import variable_names
+
+    n = args.n or DEFAULT_N[args.workload]
+    case, npk = make_case(args.workload, n)
+    nvar = len(variable_names(case))
+    H = hip()
+
+    learned_info = None
+    mlp = None
+    if "fp32_learned" in forms:
+        from ecogen_amd import learned
+        log(f"warm start: collecting about {args.samples} relaxation samples from plain FP64 runs")
+        smp, sinfo = learned.collect_samples(learned.default_sources(), target=args.samples, log=log)
+        mlp, tinfo = learned.train(smp, epochs=args.epochs, log=log)
+        smp.close()
+        learned_info = dict(samples=sinfo, training=tinfo)
+
+    solvers, res = {}, {}
+    for f in forms:
+        c = dict(case)
+        c["precision"] = f
+        log(f"{f}: building {case['name']} ({args.workload}, {npk} inclusions)")
+        t0 = time.perf_counter()
+        s = Solver(c)
+        if f == "fp32_learned":
+            s.set_warm_start(mlp)
+        s.synchronize()
+        log(f"{f}: initial condition ready in {time.perf_counter() - t0:.1f} s")
+        esize = 8 if f == "fp64" else 4
+        r = time_form(H, s, args.steps, args.warmup, esize, nvar, case["order"])
+        log(f"{f}: {args.steps} steps, {r['ms_per_step']:.2f} ms/step, {r['value']:.1f} Mcell-updates/s, "
+            f"dominant kernel {r['kind']} {r['avg_launch_s'] * 1e3:.2f} ms/launch, newton {r['newton']}")
+        if args.calibrate and f == "fp64":
+            nb = esize * nvar * s.ncell // 16 * 16
+            s.calibration_copy(nb)
+            log(f"calibration copy: {nb} bytes read and {nb} bytes written in one launch")
+        solvers[f] = s
+        res[f] = r
+
+    # Accuracy: every form from the initial condition on the FP64 path's dt sequence.
+    errors = {}
+    if not args.no_accuracy and "fp64" in forms and len(forms) > 1:
+        log("accuracy: common dt sequence from the initial condition")
+        for s in solvers.values():
+            s.refill()
+        dt = 1.e-15
+        for _ in range(args.warmup + args.steps):
+            dtm, _ = solvers["fp64"].step(dt)
+            for f in forms:
+                if f != "fp64":
+                    solvers[f].step(dt)
+            dt = case["cfl"] * dtm
+        for f in forms:
+            if f != "fp64":
+                errors[f] = {k: [float(f"{v[0]:.3e}"), float(f"{v[1]:.3e}")]
+                             for k, v in solvers[f].diff(solvers["fp64"]).items()}
+                log(f"{f}: mixture density rel L1 {errors[f]['mixture_rho'][0]:.2e}, "
+                    f"Linf {errors[f]['mixture_rho'][1]:.2e}")
+
+    head = "fp32_learned" if "fp32_learned" in forms else forms[-1]
+    hr = res[head]
+    out_forms = {}
+    for f in forms:
+        r = res[f]
+        key = f"{args.workload}:{f}:{r['kind']}"
+        nw = r["newton"]
+        out_forms[f] = {
+            "form": FORMS[f],
+            "value": round(r["value"], 3),
+            "ms_per_step": round(r["ms_per_step"], 3),
+            "speedup_vs_fp64": round(r["value"] / res["fp64"]["value"], 3) if "fp64" in res else None,
+            "rel_error_vs_fp64": errors.get(f),
+            "newton": {"evaluations": nw["newton_iters"], "relaxations": nw["relax_calls"],
+                       "per_relaxation": round(nw["newton_iters"] / max(nw["relax_calls"], 1), 5),
+                       "warm_started": nw["warm_calls"], "cold_restarts": nw["warm_fallbacks"],
+                       "interface_pressure_fallbacks": nw["newton_fail"],
+                       "mean_final_residual": nw["resid_sum"] / max(nw["relax_calls"], 1)},
+            "kernel_ms": r["kernel_ms"],
+            "roofline": {"bound": "hbm", "kernel": f"k_step ({r['kind']})", "achieved": round(r["achieved"], 2),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(r["achieved"] / HBM_PEAK_GBS, 5),
+                         "traffic": traffic_from_profiles(key), "avg_launch_ms": round(r["avg_launch_s"] * 1e3, 3),
+                         "algorithmic_bytes_per_launch": r["bytes_launch"]},
+            "whole_step_hbm_gbs": round(r["whole_step_gbs"], 2),
+        }
+    if "fp32" in res and "fp32_learned" in res:
+        saved = res["fp32"]["newton"]["newton_iters"] - res["fp32_learned"]["newton"]["newton_iters"]
+        out_forms["fp32_learned"]["newton"]["evaluations_saved_vs_fp32_cold"] = saved
     out = {
         "metric": "Mcell-updates/s: plain FP64 vs FP32+learned-relax at stated L1 err; HBM GB/s %",
-        "value": round(value, 3),
+        "value": round(hr["value"], 3),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(hr["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f32+f64",
+        "dtype": "f64" if head == "fp64" else "f32+f64",
         "data": "synthetic (seeded random air pockets, numpy default_rng 20261202)",
         "config": {
             "workload": WORKLOADS[args.workload].format(n=n, extra=npk),
-            "cells": ncell,
+            "cells": solvers[head].ncell,
             "state_vars": nvar,
-            "form": FORMS[args.form],
-            "bytes_per_cell_update": (5 if case["order"] == 2 else 2) * nvar * esize,
+            "form": FORMS[head],
+            "bytes_per_cell_update": (5 if case["order"] == 2 else 2) * nvar * (8 if head == "fp64" else 4),
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": kernel_name,
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": traffic,
-            "avg_launch_ms": round(avg_s * 1e3, 3),
-            "algorithmic_bytes_per_launch": bytes_launch,
-        },
-        "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in kt.items() if v[1]},
-        "whole_step_hbm_gbs": round((5 if case["order"] == 2 else 2) * nvar * esize * value * 1e6 / 1e9, 2),
+        "roofline": out_forms[head]["roofline"],
+        "forms": out_forms,
+        "learned": learned_info,
     }
     if not args.no_cpu_baseline:
-        print("[bench] timing the oracle on the host (bounded replica)", file=sys.stderr, flush=True)
+        log("timing the oracle on the host (bounded replica)")
         out["cpu_baseline"] = cpu_baseline(args.workload)
     else:
         out["cpu_baseline"] = None

@@ -49,7 +49,8 @@ class EcgDomain(C.Structure):
 
 class EcgStepStats(C.Structure):
     _fields_ = [("t", C.c_double), ("dt_next", C.c_double), ("dt_max", C.c_double), ("iteration", C.c_int64),
-                ("newton_fail", C.c_int64), ("newton_iters", C.c_int64), ("relax_calls", C.c_int64)]
+                ("newton_fail", C.c_int64), ("newton_iters", C.c_int64), ("relax_calls", C.c_int64),
+                ("warm_calls", C.c_int64), ("warm_fallbacks", C.c_int64), ("resid_sum", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -57,6 +58,7 @@ class EcgStepStats(C.Structure):
 
 # Exported symbols and their ctypes signatures (checked by the CPU test-suite).
 _DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
 SIGNATURES = {
     "ecg_last_error": (C.c_char_p, []),
     "ecg_version": (C.c_char_p, []),
@@ -77,7 +79,26 @@ SIGNATURES = {
     "ecg_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "ecg_kernel_times": (C.c_int, [C.c_void_p, _DP, C.POINTER(C.c_int64), C.c_int]),
     "ecg_calibration_copy": (C.c_int, [C.c_void_p, C.c_int64]),
+    "ecg_state_diff": (C.c_int, [C.c_void_p, C.c_void_p, _DP, C.c_int]),
+    "ecg_samples_create": (C.c_void_p, [C.c_int64]),
+    "ecg_samples_destroy": (None, [C.c_void_p]),
+    "ecg_samples_reset": (C.c_int, [C.c_void_p]),
+    "ecg_samples_count": (C.c_int64, [C.c_void_p]),
+    "ecg_samples_seen": (C.c_int64, [C.c_void_p]),
+    "ecg_samples_download": (C.c_int, [C.c_void_p, _FP, C.c_int64]),
+    "ecg_samples_upload": (C.c_int, [C.c_void_p, _FP, C.c_int64]),
+    "ecg_attach_samples": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "ecg_mlp_create": (C.c_void_p, [C.c_uint64]),
+    "ecg_mlp_destroy": (None, [C.c_void_p]),
+    "ecg_mlp_get_params": (C.c_int, [C.c_void_p, _FP]),
+    "ecg_mlp_set_params": (C.c_int, [C.c_void_p, _FP]),
+    "ecg_mlp_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _FP, _DP]),
+    "ecg_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint64, _DP]),
+    "ecg_mlp_predict": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _FP]),
+    "ecg_set_warm_start": (C.c_int, [C.c_void_p, C.c_void_p]),
 }
+MLP_NPARAM = 5057
+SAMPLE_FLOATS = 16
 KERNEL_KINDS = ("step", "predict", "correct", "clock")
 
 
@@ -210,6 +231,11 @@ class Solver:
     def enqueue(self, n):
         _check(self.L.ecg_enqueue_steps(self.h, int(n)))
 
+    def refill(self):
+        """Re-apply the case's initial condition (ecg_fill)."""
+        doms = make_domains(self.case)
+        _check(self.L.ecg_fill(self.h, doms, len(doms)))
+
     def synchronize(self):
         _check(self.L.ecg_synchronize(self.h, C.byref(self.stats)))
         return self.stats.as_dict()
@@ -230,3 +256,125 @@ class Solver:
     @property
     def stream(self):
         return self.L.ecg_stream(self.h)
+
+    def diff(self, ref):
+        """{field: (rel L1, rel Linf)} of this state against `ref` (same mesh, any form); the
+        relative L-infinity error is max|a-b| / max|b|; field 'mixture_rho' last."""
+        from .case import variable_names
+        nf = self.nvar + 1
+        out = np.zeros(4 * nf)
+        _check(self.L.ecg_state_diff(self.h, ref.h, out.ctypes.data_as(_DP), nf))
+        names = variable_names(self.case) + ["mixture_rho"]
+        res = {}
+        for f, nm in enumerate(names):
+            sd, sb = out[4 * f], out[4 * f + 1]
+            md, mb = out[4 * f + 2:4 * f + 4].copy().view(np.float64)
+            res[nm] = (sd / sb if sb > 0 else float(sd > 0) * np.inf, md / mb if mb > 0 else float(md > 0) * np.inf)
+        return res
+
+    def attach_samples(self, samples, stride=1):
+        """Append relaxation samples of the following steps to `samples` (None detaches)."""
+        _check(self.L.ecg_attach_samples(self.h, samples.h if samples is not None else None, int(stride)))
+
+    def set_warm_start(self, mlp):
+        """Relax from the network's guess (fp32_learned cases; None: cold start)."""
+        _check(self.L.ecg_set_warm_start(self.h, mlp.h if mlp is not None else None))
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+class Samples:
+    """Device buffer of relaxation samples (records of SAMPLE_FLOATS floats)."""
+
+    def __init__(self, capacity):
+        self.L = library()
+        self.h = self.L.ecg_samples_create(int(capacity))
+        if not self.h:
+            raise EcgError(self.L.ecg_last_error().decode())
+        self.capacity = int(capacity)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ecg_samples_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return int(_check(self.L.ecg_samples_count(self.h)))
+
+    def reset(self):
+        _check(self.L.ecg_samples_reset(self.h))
+
+    @property
+    def seen(self):
+        """Candidate cells met while attached."""
+        return int(_check(self.L.ecg_samples_seen(self.h)))
+
+    def download(self, n=None):
+        n = len(self) if n is None else int(n)
+        out = np.empty((n, SAMPLE_FLOATS), dtype=np.float32)
+        _check(self.L.ecg_samples_download(self.h, _fp(out), n))
+        return out
+
+    def upload(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=np.float32)
+        if rec.ndim != 2 or rec.shape[1] != SAMPLE_FLOATS:
+            raise ValueError(f"records must have shape (n, {SAMPLE_FLOATS})")
+        _check(self.L.ecg_samples_upload(self.h, _fp(rec), rec.shape[0]))
+
+
+class Mlp:
+    """The warm-start network on the device (FP32 parameters, BF16 MFMA inference)."""
+
+    def __init__(self, seed=1):
+        self.L = library()
+        self.h = self.L.ecg_mlp_create(int(seed))
+        if not self.h:
+            raise EcgError(self.L.ecg_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ecg_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.empty(MLP_NPARAM, dtype=np.float32)
+        _check(self.L.ecg_mlp_get_params(self.h, _fp(out)))
+        return out
+
+    def set_params(self, prm):
+        prm = np.ascontiguousarray(prm, dtype=np.float32)
+        if prm.shape != (MLP_NPARAM,):
+            raise ValueError(f"parameters must have shape ({MLP_NPARAM},)")
+        _check(self.L.ecg_mlp_set_params(self.h, _fp(prm)))
+
+    def gradient(self, samples, n):
+        g = np.empty(MLP_NPARAM, dtype=np.float32)
+        loss = np.zeros(1)
+        _check(self.L.ecg_mlp_gradient(self.h, samples.h, int(n), _fp(g), loss.ctypes.data_as(_DP)))
+        return g, float(loss[0])
+
+    def train(self, samples, epochs=4, batch=4096, lr=1e-3, seed=1):
+        losses = np.zeros(int(epochs))
+        _check(self.L.ecg_mlp_train(self.h, samples.h, int(epochs), int(batch), float(lr), int(seed),
+                                    losses.ctypes.data_as(_DP)))
+        return losses
+
+    def predict(self, samples, n=None):
+        n = len(samples) if n is None else int(n)
+        out = np.empty(n, dtype=np.float32)
+        _check(self.L.ecg_mlp_predict(self.h, samples.h, n, _fp(out)))
+        return out

@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libecogen_amd.so")
 ARCH = os.environ.get("ECG_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
-SOURCES = ["ecg_api.hip"] + [f"ecg_step_{m}_{p}.hip" for m in ("euler", "pueq", "ueq") for p in ("f64", "f32")]
+SOURCES = ["ecg_api.hip", "ecg_mlp.hip"] + [f"ecg_step_{m}_{p}.hip" for m in ("euler", "pueq", "ueq") for p in ("f64", "f32")]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -10,6 +10,7 @@
 
 #include "../../include/ecogen_amd.h"
 #include "ecg_kernels.hpp"
+#include "ecg_learn.hpp"
 
 namespace ecg {
 step_fn get_step_fn_euler_f64(int nph, int dim);
@@ -74,6 +75,57 @@ __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, d
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
 }
 
+// Field-wise differences of two states of one mesh: for every persistent
+// variable and, last, the mixture density sum_k alpha_k rho_k (Euler: rho),
+// out[4f..4f+3] accumulates sum|a-b|, sum|b|, max|a-b|, max|b| (maxima as
+// the bits of non-negative doubles).
+template <typename SA, typename SB>
+__global__ __launch_bounds__(256) void k_state_diff(const SA* __restrict__ a, const SB* __restrict__ b, int64_t ncell,
+                                                    int64_t pa, int64_t pb, int nvar, int nph, int multiphase,
+                                                    double* __restrict__ out) {
+  __shared__ double red[4][4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int f = 0; f <= nvar; f++) {
+    double sd = 0., sb = 0., md = 0., mb = 0.;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += stride) {
+      double va, vb;
+      if (f < nvar) {
+        va = (double)a[f * pa + i];
+        vb = (double)b[f * pb + i];
+      } else if (multiphase) {
+        va = 0.; vb = 0.;
+        for (int k = 0; k < nph; k++) {
+          va += (double)a[(3 * k) * pa + i] * (double)a[(3 * k + 1) * pa + i];
+          vb += (double)b[(3 * k) * pb + i] * (double)b[(3 * k + 1) * pb + i];
+        }
+      } else {
+        va = (double)a[i];
+        vb = (double)b[i];
+      }
+      const double d = fabs(va - vb), ab = fabs(vb);
+      sd += d; sb += ab; md = fmax(md, d); mb = fmax(mb, ab);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      sd += __shfl_xor(sd, o, 64); sb += __shfl_xor(sb, o, 64);
+      md = fmax(md, __shfl_xor(md, o, 64)); mb = fmax(mb, __shfl_xor(mb, o, 64));
+    }
+    if (lane == 0) { red[wv][0] = sd; red[wv][1] = sb; red[wv][2] = md; red[wv][3] = mb; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < 4; w++) {
+        sd += red[w][0]; sb += red[w][1]; md = fmax(md, red[w][2]); mb = fmax(mb, red[w][3]);
+      }
+      atomicAdd(&out[4 * f + 0], sd);
+      atomicAdd(&out[4 * f + 1], sb);
+      atomicMax(reinterpret_cast<unsigned long long*>(&out[4 * f + 2]), (unsigned long long)__double_as_longlong(md));
+      atomicMax(reinterpret_cast<unsigned long long*>(&out[4 * f + 3]), (unsigned long long)__double_as_longlong(mb));
+    }
+    __syncthreads();
+  }
+}
+
 void launch_clock(Clock* clk, double cfl, hipStream_t s) { hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, clk, cfl); }
 
 }  // namespace ecg
@@ -86,6 +138,10 @@ static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
+namespace ecg {
+int api_fail(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace ecg
 
 #define HIPCHK(expr)                                                                                   \
   do {                                                                                                 \
@@ -117,6 +173,8 @@ struct ecg_ctx {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   double kt_ms[ECG_K_KINDS] = {0, 0, 0, 0};
   int64_t kt_n[ECG_K_KINDS] = {0, 0, 0, 0};
+  MlpDev mlp = {nullptr, nullptr};                // learned warm start (ecg_set_warm_start)
+  SampleSink sink = {nullptr, nullptr, 0, 0, 1};  // relaxation sample capture (ecg_attach_samples)
 };
 
 static hipEvent_t take_event(ecg_ctx* c) {
@@ -213,8 +271,10 @@ static int check_config(const ecg_config* c) {
       return fail(ECG_E_UNSUPPORTED, "second-order wall/symmetry boundaries (BoundCondWallO2Cartesian) are not built");
   }
   if (!(c->cfl > 0)) return fail(ECG_E_ARG, "CFL must be positive");
-  if (c->precision != ECG_FP64 && c->precision != ECG_FP32_RELAX64)
-    return fail(ECG_E_UNSUPPORTED, "learned warm start is not available in this build");
+  if (c->precision != ECG_FP64 && c->precision != ECG_FP32_RELAX64 && c->precision != ECG_FP32_LEARNED)
+    return fail(ECG_E_ARG, "unknown precision form");
+  if (c->precision == ECG_FP32_LEARNED && (c->model == ECG_EULER || (c->model == ECG_UEQ && !c->relax_p)))
+    return fail(ECG_E_ARG, "the learned warm start needs a relaxing multiphase model");
   return ECG_OK;
 }
 
@@ -453,6 +513,8 @@ int ecg_download_state(ecg_ctx* c, double* const* soa, int nvars) {
 static int enqueue_one(ecg_ctx* c) {
   const int nxt = 1 - c->cur;
   StepLaunch L;
+  L.mlp = c->mlp;
+  L.sink = c->sink;
   if (c->cfg.order == 1) {
     L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[nxt]; L.stage = 0;
     timed(c, ECG_K_STEP, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream); });
@@ -498,6 +560,9 @@ static void fill_stats(ecg_ctx* c, ecg_step_stats* st) {
   st->newton_fail = (int64_t)c->hcnt->newton_fail;
   st->newton_iters = (int64_t)c->hcnt->newton_iters;
   st->relax_calls = (int64_t)c->hcnt->relax_calls;
+  st->warm_calls = (int64_t)c->hcnt->warm_calls;
+  st->warm_fallbacks = (int64_t)c->hcnt->warm_fallbacks;
+  st->resid_sum = c->hcnt->resid_sum;
 }
 
 int ecg_step(ecg_ctx* c, double dt, double* dtmax_out, ecg_step_stats* st) {
@@ -584,6 +649,76 @@ int ecg_calibration_copy(ecg_ctx* c, int64_t nbytes) {
                      (double2*)c->W[1 - c->cur], n2);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
+  return ECG_OK;
+}
+
+int ecg_state_diff(ecg_ctx* a, ecg_ctx* b, double* out, int nfields) {
+  if (!a || !b || !out) return fail(ECG_E_ARG, "null argument");
+  if (a->ncell != b->ncell || a->nvar != b->nvar || a->cfg.model != b->cfg.model || a->cfg.nph != b->cfg.nph ||
+      a->cfg.nx != b->cfg.nx || a->cfg.ny != b->cfg.ny || a->cfg.nz != b->cfg.nz)
+    return fail(ECG_E_ARG, "the two contexts hold different meshes or models");
+  if (nfields != a->nvar + 1) return fail(ECG_E_ARG, "nfields must be nvar + 1");
+  int rc = settle(a);
+  if (rc != ECG_OK) return rc;
+  rc = settle(b);
+  if (rc != ECG_OK) return rc;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  double* d = nullptr;
+  const size_t bytes = sizeof(double) * 4 * (size_t)nfields;
+  HIPCHK(hipMalloc(&d, bytes));
+  hipError_t e = hipMemsetAsync(d, 0, bytes, a->stream);
+  const int blocks = (int)std::min<int64_t>((a->ncell + 255) / 256, 4096);
+  const int mp = a->cfg.model != ECG_EULER;
+  if (e == hipSuccess) {
+    if (a->esize == 8 && b->esize == 8)
+      hipLaunchKernelGGL((k_state_diff<double, double>), dim3(blocks), dim3(256), 0, a->stream, (const double*)a->W[a->cur],
+                         (const double*)b->W[b->cur], a->ncell, a->pitch, b->pitch, a->nvar, a->cfg.nph, mp, d);
+    else if (a->esize == 4 && b->esize == 8)
+      hipLaunchKernelGGL((k_state_diff<float, double>), dim3(blocks), dim3(256), 0, a->stream, (const float*)a->W[a->cur],
+                         (const double*)b->W[b->cur], a->ncell, a->pitch, b->pitch, a->nvar, a->cfg.nph, mp, d);
+    else if (a->esize == 8 && b->esize == 4)
+      hipLaunchKernelGGL((k_state_diff<double, float>), dim3(blocks), dim3(256), 0, a->stream, (const double*)a->W[a->cur],
+                         (const float*)b->W[b->cur], a->ncell, a->pitch, b->pitch, a->nvar, a->cfg.nph, mp, d);
+    else
+      hipLaunchKernelGGL((k_state_diff<float, float>), dim3(blocks), dim3(256), 0, a->stream, (const float*)a->W[a->cur],
+                         (const float*)b->W[b->cur], a->ncell, a->pitch, b->pitch, a->nvar, a->cfg.nph, mp, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, a->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
+  hipFree(d);
+  if (e != hipSuccess) return fail(ECG_E_HIP, std::string("state comparison: ") + hipGetErrorString(e));
+  return ECG_OK;
+}
+
+int ecg_attach_samples(ecg_ctx* c, ecg_samples* smp, int stride) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  if (!smp) {
+    c->sink = SampleSink{nullptr, nullptr, 0, 0, 1};
+    return ECG_OK;
+  }
+  if (stride < 1) return fail(ECG_E_ARG, "sample stride must be at least 1");
+  if (c->cfg.model == ECG_EULER || !c->cfg.relax_p) return fail(ECG_E_ARG, "the case has no pressure relaxation");
+  // Records written so far stay; the candidate counter restarts for this attachment.
+  const long long held = samples_count(smp);
+  if (held < 0) return fail(ECG_E_HIP, "reading the sample counter failed");
+  HIPCHK(hipMemset(smp->seen, 0, sizeof(unsigned long long)));
+  smp->base = held;
+  smp->stride = stride;
+  smp->uploaded = -1;
+  c->sink = SampleSink{smp->buf, smp->seen, smp->cap, held, stride};
+  return ECG_OK;
+}
+
+int ecg_set_warm_start(ecg_ctx* c, const ecg_mlp* m) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  if (!m) {
+    c->mlp = MlpDev{nullptr, nullptr};
+    return ECG_OK;
+  }
+  if (c->cfg.precision != ECG_FP32_LEARNED) return fail(ECG_E_ARG, "the warm start belongs to the ECG_FP32_LEARNED form");
+  HIPCHK(hipStreamSynchronize(m->stream));
+  c->mlp = MlpDev{m->frag, m->bias};
   return ECG_OK;
 }
 

@@ -2,7 +2,7 @@
 // per-model translation units and the C ABI layer.
 #pragma once
 
-#include "ecg_physics.hpp"
+#include "ecg_mlp.hpp"
 
 namespace ecg {
 
@@ -17,12 +17,24 @@ struct DomainDev {
   double vel[3];
 };
 
+// Relaxation samples written while a sink is attached: every stride-th cell
+// that relaxes from distinct phase pressures, up to cap records.
+struct SampleSink {
+  float* buf;                  // cap * SAMPLE_FLOATS
+  unsigned long long* seen;    // candidates met since the attachment
+  long long cap;
+  long long base;              // records held before the attachment
+  int stride;
+};
+
 // Buffers of one launch; the element type is the state type of the precision form.
 struct StepLaunch {
   const void* wn;       // state at time n
   const void* wsrc;     // state the fluxes are evaluated on (== wn for first order / predictor)
   void* wout;           // written state
   int stage;            // 0: first order; 1: predictor; 2: corrector
+  MlpDev mlp;           // learned warm start (frag == nullptr: cold start)
+  SampleSink sink;      // relaxation sample capture (buf == nullptr: off)
 };
 
 template <typename R>
@@ -31,6 +43,8 @@ struct StepLaunchT {
   const R* wsrc;
   R* wout;
   int stage;
+  MlpDev mlp;
+  SampleSink sink;
 };
 
 // Per (model, phases, dimension, precision) launchers; defined in ecg_step_<model>_<precision>.hip.

@@ -1,0 +1,515 @@
+// Learned Newton warm start: relaxation sample buffers, training kernels
+// (FP32 forward/backward with deterministic block partials, Adam) and the
+// BF16 fragment packing used by the fused MFMA inference in the step kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "ecg_learn.hpp"
+
+namespace ecg {
+
+constexpr int TBS = 128;   // samples per training block
+constexpr int LDP = 65;    // padded row of the block's activation matrices
+constexpr int PRM_PAD = (MLP_NPARAM + 3) & ~3;
+constexpr size_t TRAIN_LDS_FLOATS = PRM_PAD + 3 * TBS * LDP + TBS * 13 + 2 * TBS;
+
+// Orders of the three phase slots; training sees every order of every sample.
+__constant__ int k_slot_perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Mean squared error of the remainder r = t - t_lin over `count` samples taken
+// as idx = (mult * (q0 + b) + offs) mod n; per-block parameter gradients of
+// inv_batch * sum (y - r)^2 and per-block loss sums.
+__global__ __launch_bounds__(TBS) void k_mlp_fwd_bwd(const float* __restrict__ prm_g, const float* __restrict__ smp,
+                                                     long long n, long long q0, long long count,
+                                                     unsigned long long mult, unsigned long long offs,
+                                                     unsigned long long perm_seed, float inv_batch,
+                                                     float* __restrict__ gpart, float* __restrict__ lpart) {
+  extern __shared__ __align__(16) float sm[];
+  float* prm = sm;
+  float* H1 = prm + PRM_PAD;
+  float* DZ2 = H1 + TBS * LDP;
+  float* G3 = DZ2 + TBS * LDP;
+  float* X = G3 + TBS * LDP;
+  float* DY = X + TBS * 13;
+  float* LS = DY + TBS;
+  const int t = threadIdx.x;
+  for (int i = t; i < MLP_NPARAM; i += TBS) prm[i] = prm_g[i];
+
+  float x[MLP_IN];
+#pragma unroll
+  for (int i = 0; i < MLP_IN; i++) x[i] = 0.f;
+  float r = 0.f, msk = 0.f;
+  const long long b = (long long)blockIdx.x * TBS + t;
+  if (b < count) {
+    const unsigned long long q = (unsigned long long)(q0 + b);
+    const long long idx = (long long)((mult * q + offs) % (unsigned long long)n);
+    const float* rec = smp + idx * SAMPLE_FLOATS;
+    const int pm = perm_seed ? (int)(mix64(perm_seed ^ (unsigned long long)idx * 0x9E3779B97F4A7C15ull) % 6ull) : 0;
+#pragma unroll
+    for (int sl = 0; sl < MLP_SLOTS; sl++) {
+      const int src = k_slot_perm[pm][sl];
+#pragma unroll
+      for (int f = 0; f < 4; f++) x[4 * sl + f] = rec[4 * src + f];
+    }
+    float tl = 0.f;
+#pragma unroll
+    for (int k = 0; k < MLP_SLOTS; k++) tl += x[4 * k + 2] * x[4 * k + 1];
+    r = rec[12] - tl;
+    msk = 1.f;
+  }
+  __syncthreads();
+  const float* W1 = prm + MLP_OFF_W1;
+  const float* B1 = prm + MLP_OFF_B1;
+  const float* W2 = prm + MLP_OFF_W2;
+  const float* B2 = prm + MLP_OFF_B2;
+  const float* W3 = prm + MLP_OFF_W3;
+  const float B3 = prm[MLP_OFF_B3];
+
+  float z1[MLP_H], h1[MLP_H], z2[MLP_H];
+#pragma unroll
+  for (int j = 0; j < MLP_H; j++) {
+    float a = B1[j];
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) a += x[i] * W1[i * MLP_H + j];
+    z1[j] = a;
+    h1[j] = gelu_f(a);
+  }
+#pragma unroll
+  for (int j = 0; j < MLP_H; j++) {
+    float a = B2[j];
+#pragma unroll
+    for (int i = 0; i < MLP_H; i++) a += h1[i] * W2[i * MLP_H + j];
+    z2[j] = a;
+  }
+  float y = B3;
+#pragma unroll
+  for (int j = 0; j < MLP_H; j++) y += W3[j] * gelu_f(z2[j]);
+  const float d = y - r;
+  const float loss = msk * d * d;
+  const float dy = msk * 2.f * d * inv_batch;
+#pragma unroll
+  for (int j = 0; j < MLP_H; j++) {
+    H1[t * LDP + j] = h1[j];
+    G3[t * LDP + j] = dy * gelu_f(z2[j]);
+    const float g = dy * W3[j] * gelu_grad_f(z2[j]);
+    DZ2[t * LDP + j] = g;
+    z2[j] = g;
+  }
+#pragma unroll
+  for (int i = 0; i < MLP_H; i++) {
+    float a = 0.f;
+#pragma unroll
+    for (int j = 0; j < MLP_H; j++) a += W2[i * MLP_H + j] * z2[j];
+    z1[i] = a * gelu_grad_f(z1[i]);
+  }
+  DY[t] = dy;
+  LS[t] = loss;
+#pragma unroll
+  for (int i = 0; i < MLP_IN; i++) X[t * 13 + i] = x[i];
+  __syncthreads();
+
+  float* gp = gpart + (size_t)blockIdx.x * MLP_NPARAM;
+  for (int e = t; e < MLP_H * MLP_H; e += TBS) {
+    const int i = e >> 6, j = e & 63;
+    float a = 0.f;
+    for (int s = 0; s < TBS; s++) a += H1[s * LDP + i] * DZ2[s * LDP + j];
+    gp[MLP_OFF_W2 + e] = a;
+  }
+  if (t < MLP_H) {
+    float a = 0.f, c = 0.f;
+    for (int s = 0; s < TBS; s++) { a += DZ2[s * LDP + t]; c += G3[s * LDP + t]; }
+    gp[MLP_OFF_B2 + t] = a;
+    gp[MLP_OFF_W3 + t] = c;
+  } else if (t == MLP_H) {
+    float a = 0.f;
+    for (int s = 0; s < TBS; s++) a += DY[s];
+    gp[MLP_OFF_B3] = a;
+  } else if (t == MLP_H + 1) {
+    float a = 0.f;
+    for (int s = 0; s < TBS; s++) a += LS[s];
+    lpart[blockIdx.x] = a;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MLP_H; i++) H1[t * LDP + i] = z1[i];   // dL/dz1
+  __syncthreads();
+  for (int e = t; e < MLP_IN * MLP_H; e += TBS) {
+    const int i = e >> 6, j = e & 63;
+    float a = 0.f;
+    for (int s = 0; s < TBS; s++) a += X[s * 13 + i] * H1[s * LDP + j];
+    gp[MLP_OFF_W1 + e] = a;
+  }
+  if (t < MLP_H) {
+    float a = 0.f;
+    for (int s = 0; s < TBS; s++) a += H1[s * LDP + t];
+    gp[MLP_OFF_B1 + t] = a;
+  }
+}
+
+// Fixed-order sum of the block partials (deterministic gradients).
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ gpart, int nblk, float* __restrict__ grad) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= MLP_NPARAM) return;
+  float s = 0.f;
+  for (int b = 0; b < nblk; b++) s += gpart[(size_t)b * MLP_NPARAM + p];
+  grad[p] = s;
+}
+
+__global__ void k_mlp_loss(const float* __restrict__ lpart, int nblk, double* __restrict__ acc) {
+  if (threadIdx.x != 0) return;
+  double s = 0.;
+  for (int b = 0; b < nblk; b++) s += (double)lpart[b];
+  *acc += s;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ prm, const float* __restrict__ grad,
+                                                  float* __restrict__ m, float* __restrict__ v, float lr_t) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= MLP_NPARAM) return;
+  const float g = grad[p];
+  const float mm = 0.9f * m[p] + 0.1f * g;
+  const float vv = 0.999f * v[p] + 0.001f * g * g;
+  m[p] = mm;
+  v[p] = vv;
+  prm[p] -= lr_t * mm / (sqrtf(vv) + 1e-8f);
+}
+
+// BF16 A fragments of v_mfma_f32_16x16x16bf16_1k (lane l holds A[l%16][4*(l/16)+j])
+// of W^T for every (output tile, input tile), plus the FP32 biases.
+__global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ prm, uint2* __restrict__ frag,
+                                                  float* __restrict__ bias) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e < MLP_NBIAS) {
+    bias[e] = (e < MLP_H) ? prm[MLP_OFF_B1 + e] : (e < 2 * MLP_H) ? prm[MLP_OFF_B2 + e - MLP_H] : prm[MLP_OFF_B3];
+  }
+  if (e >= MLP_NFRAG * 64) return;
+  const int f = e >> 6, l = e & 63;
+  const int i = l & 15, k0 = 4 * (l >> 4);
+  float v[4];
+  for (int j = 0; j < 4; j++) {
+    const int k = k0 + j;
+    float a = 0.f;
+    if (f < MLP_FRAG_W2) {
+      const int mt = f - MLP_FRAG_W1;
+      if (k < MLP_IN) a = prm[MLP_OFF_W1 + k * MLP_H + mt * 16 + i];
+    } else if (f < MLP_FRAG_W3) {
+      const int mt = (f - MLP_FRAG_W2) >> 2, kt = (f - MLP_FRAG_W2) & 3;
+      a = prm[MLP_OFF_W2 + (kt * 16 + k) * MLP_H + mt * 16 + i];
+    } else {
+      const int kt = f - MLP_FRAG_W3;
+      a = (i == 0) ? prm[MLP_OFF_W3 + kt * 16 + k] : 0.f;
+    }
+    v[j] = a;
+  }
+  frag[e] = make_uint2(pack_bf16x2(v[0], v[1]), pack_bf16x2(v[2], v[3]));
+}
+
+// Stand-alone inference over sample records with the device function the step
+// kernel fuses into the relaxation: out = t_lin + y.
+__global__ __launch_bounds__(256) void k_mlp_predict(MlpDev M, const float* __restrict__ smp, long long n,
+                                                     float* __restrict__ out) {
+  __shared__ __align__(16) unsigned char stage[4 * (64 * 16 * 2 + 64 * 4)];
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  float x[MLP_IN];
+#pragma unroll
+  for (int i = 0; i < MLP_IN; i++) x[i] = 0.f;
+  float tl = 0.f;
+  if (g < n) {
+    const float* rec = smp + g * SAMPLE_FLOATS;
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) x[i] = rec[i];
+#pragma unroll
+    for (int k = 0; k < MLP_SLOTS; k++) tl += x[4 * k + 2] * x[4 * k + 1];
+  }
+  const float y = mlp_infer_wave(M, x, reinterpret_cast<unsigned short*>(stage + (threadIdx.x >> 6) * (64 * 16 * 2 + 64 * 4)));
+  if (g < n) out[g] = tl + y;
+}
+
+int api_fail(int code, const std::string& msg);
+
+long long samples_count(ecg_samples* s) {
+  if (s->uploaded >= 0) return s->uploaded;
+  unsigned long long seen = 0;
+  if (hipDeviceSynchronize() != hipSuccess) return -1;
+  if (hipMemcpy(&seen, s->seen, sizeof(seen), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  const unsigned long long st = (unsigned long long)(s->stride > 0 ? s->stride : 1);
+  const unsigned long long used = (unsigned long long)s->base + (seen + st - 1) / st;
+  return (long long)(used < (unsigned long long)s->cap ? used : (unsigned long long)s->cap);
+}
+
+}  // namespace ecg
+
+using namespace ecg;
+
+#define HIPCHK(expr)                                                                                     \
+  do {                                                                                                   \
+    hipError_t e_ = (expr);                                                                              \
+    if (e_ != hipSuccess) return api_fail(ECG_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" {
+
+ecg_samples* ecg_samples_create(int64_t capacity) {
+  if (capacity <= 0) { api_fail(ECG_E_ARG, "sample capacity must be positive"); return nullptr; }
+  ecg_samples* s = new ecg_samples();
+  s->cap = capacity;
+  if (hipMalloc(&s->buf, (size_t)capacity * SAMPLE_FLOATS * sizeof(float)) != hipSuccess ||
+      hipMalloc(&s->seen, sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(s->seen, 0, sizeof(unsigned long long)) != hipSuccess) {
+    api_fail(ECG_E_HIP, "device allocation of the sample buffer failed");
+    if (s->buf) hipFree(s->buf);
+    if (s->seen) hipFree(s->seen);
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+void ecg_samples_destroy(ecg_samples* s) {
+  if (!s) return;
+  hipDeviceSynchronize();
+  hipFree(s->buf);
+  hipFree(s->seen);
+  delete s;
+}
+
+int ecg_samples_reset(ecg_samples* s) {
+  if (!s) return api_fail(ECG_E_ARG, "null sample buffer");
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemset(s->seen, 0, sizeof(unsigned long long)));
+  s->uploaded = -1;
+  s->base = 0;
+  s->stride = 1;
+  return ECG_OK;
+}
+
+int64_t ecg_samples_count(ecg_samples* s) {
+  if (!s) return api_fail(ECG_E_ARG, "null sample buffer");
+  const long long n = samples_count(s);
+  if (n < 0) return api_fail(ECG_E_HIP, "reading the sample counter failed");
+  return n;
+}
+
+int64_t ecg_samples_seen(ecg_samples* s) {
+  if (!s) return api_fail(ECG_E_ARG, "null sample buffer");
+  unsigned long long seen = 0;
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(&seen, s->seen, sizeof(seen), hipMemcpyDeviceToHost));
+  return (int64_t)seen;
+}
+
+int ecg_samples_download(ecg_samples* s, float* host, int64_t n) {
+  if (!s || !host || n < 0) return api_fail(ECG_E_ARG, "invalid sample download arguments");
+  const long long have = samples_count(s);
+  if (have < 0) return api_fail(ECG_E_HIP, "reading the sample counter failed");
+  if (n > have) return api_fail(ECG_E_ARG, "more records requested than the buffer holds");
+  HIPCHK(hipMemcpy(host, s->buf, (size_t)n * SAMPLE_FLOATS * sizeof(float), hipMemcpyDeviceToHost));
+  return ECG_OK;
+}
+
+int ecg_samples_upload(ecg_samples* s, const float* host, int64_t n) {
+  if (!s || !host || n < 0 || n > s->cap) return api_fail(ECG_E_ARG, "invalid sample upload arguments");
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(s->buf, host, (size_t)n * SAMPLE_FLOATS * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(s->seen, 0, sizeof(unsigned long long)));
+  s->uploaded = n;
+  return ECG_OK;
+}
+
+ecg_mlp* ecg_mlp_create(uint64_t seed) {
+  ecg_mlp* M = new ecg_mlp();
+  bool ok = hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && hipMalloc(&M->prm, MLP_NPARAM * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc(&M->adam_m, MLP_NPARAM * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc(&M->adam_v, MLP_NPARAM * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc(&M->grad, MLP_NPARAM * sizeof(float)) == hipSuccess;
+  ok = ok && hipMalloc(&M->loss_acc, sizeof(double)) == hipSuccess;
+  ok = ok && hipMalloc(&M->frag, MLP_NFRAG * 64 * sizeof(uint2)) == hipSuccess;
+  ok = ok && hipMalloc(&M->bias, MLP_NBIAS * sizeof(float)) == hipSuccess;
+  if (!ok) {
+    api_fail(ECG_E_HIP, "device allocation of the network failed");
+    ecg_mlp_destroy(M);
+    return nullptr;
+  }
+  // He-scaled normal weights (xorshift64* + Box-Muller), zero biases, small output layer
+  // so that the untrained network returns the linearised guess.
+  std::vector<float> h(MLP_NPARAM, 0.f);
+  unsigned long long st = seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+  auto uni = [&]() {
+    st ^= st >> 12; st ^= st << 25; st ^= st >> 27;
+    return ((st * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+  };
+  auto normal = [&]() {
+    double u1 = uni(), u2 = uni();
+    if (u1 < 1e-300) u1 = 1e-300;
+    return std::sqrt(-2. * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  };
+  for (int i = 0; i < MLP_IN * MLP_H; i++) h[MLP_OFF_W1 + i] = (float)(normal() * std::sqrt(2. / MLP_IN));
+  for (int i = 0; i < MLP_H * MLP_H; i++) h[MLP_OFF_W2 + i] = (float)(normal() * std::sqrt(2. / MLP_H));
+  for (int i = 0; i < MLP_H; i++) h[MLP_OFF_W3 + i] = (float)(normal() * 0.01);
+  if (ecg_mlp_set_params(M, h.data()) != ECG_OK) {
+    ecg_mlp_destroy(M);
+    return nullptr;
+  }
+  return M;
+}
+
+void ecg_mlp_destroy(ecg_mlp* M) {
+  if (!M) return;
+  if (M->stream) hipStreamSynchronize(M->stream);
+  hipFree(M->prm); hipFree(M->adam_m); hipFree(M->adam_v); hipFree(M->grad);
+  hipFree(M->gpart); hipFree(M->lpart); hipFree(M->loss_acc); hipFree(M->frag); hipFree(M->bias);
+  if (M->stream) hipStreamDestroy(M->stream);
+  delete M;
+}
+
+static int mlp_pack(ecg_mlp* M) {
+  hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_NFRAG * 64 + 255) / 256), dim3(256), 0, M->stream, M->prm, M->frag, M->bias);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(M->stream));
+  return ECG_OK;
+}
+
+int ecg_mlp_set_params(ecg_mlp* M, const float* host) {
+  if (!M || !host) return api_fail(ECG_E_ARG, "null network or parameters");
+  for (int i = 0; i < MLP_NPARAM; i++)
+    if (!std::isfinite(host[i])) return api_fail(ECG_E_ARG, "non-finite network parameter");
+  HIPCHK(hipMemcpy(M->prm, host, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(M->adam_m, 0, MLP_NPARAM * sizeof(float)));
+  HIPCHK(hipMemset(M->adam_v, 0, MLP_NPARAM * sizeof(float)));
+  M->adam_t = 0;
+  return mlp_pack(M);
+}
+
+int ecg_mlp_get_params(ecg_mlp* M, float* host) {
+  if (!M || !host) return api_fail(ECG_E_ARG, "null network or parameters");
+  HIPCHK(hipStreamSynchronize(M->stream));
+  HIPCHK(hipMemcpy(host, M->prm, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  return ECG_OK;
+}
+
+static int ensure_partials(ecg_mlp* M, int nblk) {
+  if (nblk <= M->gpart_blocks) return ECG_OK;
+  HIPCHK(hipStreamSynchronize(M->stream));
+  hipFree(M->gpart);
+  hipFree(M->lpart);
+  M->gpart = nullptr;
+  M->lpart = nullptr;
+  M->gpart_blocks = 0;
+  HIPCHK(hipMalloc(&M->gpart, (size_t)nblk * MLP_NPARAM * sizeof(float)));
+  HIPCHK(hipMalloc(&M->lpart, (size_t)nblk * sizeof(float)));
+  M->gpart_blocks = nblk;
+  return ECG_OK;
+}
+
+static int launch_fwd_bwd(ecg_mlp* M, const ecg_samples* S, long long n, long long q0, long long count,
+                          unsigned long long mult, unsigned long long offs, unsigned long long perm_seed,
+                          float inv_batch, int nblk) {
+  const size_t lds = TRAIN_LDS_FLOATS * sizeof(float);
+  HIPCHK(hipFuncSetAttribute((const void*)k_mlp_fwd_bwd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nblk), dim3(TBS), lds, M->stream, M->prm, S->buf, n, q0, count, mult,
+                     offs, perm_seed, inv_batch, M->gpart, M->lpart);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, M->stream, M->gpart, nblk, M->grad);
+  hipLaunchKernelGGL(k_mlp_loss, dim3(1), dim3(64), 0, M->stream, M->lpart, nblk, M->loss_acc);
+  HIPCHK(hipGetLastError());
+  return ECG_OK;
+}
+
+int ecg_mlp_gradient(ecg_mlp* M, ecg_samples* S, int64_t n, float* grad, double* loss) {
+  if (!M || !S || !grad || n <= 0) return api_fail(ECG_E_ARG, "invalid gradient arguments");
+  const long long have = samples_count(S);
+  if (have < 0) return api_fail(ECG_E_HIP, "reading the sample counter failed");
+  if (n > have) return api_fail(ECG_E_ARG, "more records requested than the buffer holds");
+  const int nblk = (int)((n + TBS - 1) / TBS);
+  int rc = ensure_partials(M, nblk);
+  if (rc != ECG_OK) return rc;
+  HIPCHK(hipMemsetAsync(M->loss_acc, 0, sizeof(double), M->stream));
+  rc = launch_fwd_bwd(M, S, n, 0, n, 1ull, 0ull, 0ull, 1.f / (float)n, nblk);
+  if (rc != ECG_OK) return rc;
+  double acc = 0.;
+  HIPCHK(hipMemcpyAsync(grad, M->grad, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost, M->stream));
+  HIPCHK(hipMemcpyAsync(&acc, M->loss_acc, sizeof(double), hipMemcpyDeviceToHost, M->stream));
+  HIPCHK(hipStreamSynchronize(M->stream));
+  if (loss) *loss = acc / (double)n;
+  return ECG_OK;
+}
+
+static unsigned long long gcd_u(unsigned long long a, unsigned long long b) {
+  while (b) { const unsigned long long t = a % b; a = b; b = t; }
+  return a;
+}
+
+int ecg_mlp_train(ecg_mlp* M, ecg_samples* S, int epochs, int batch, float lr, uint64_t seed, double* epoch_loss) {
+  if (!M || !S || epochs <= 0 || batch < TBS || !(lr > 0.f)) return api_fail(ECG_E_ARG, "invalid training arguments");
+  batch = batch / TBS * TBS;
+  const long long n = samples_count(S);
+  if (n < 0) return api_fail(ECG_E_HIP, "reading the sample counter failed");
+  if (n < batch) return api_fail(ECG_E_ARG, "fewer samples than one batch");
+  if (n >= (1ll << 31)) return api_fail(ECG_E_ARG, "at most 2^31 samples per training call");
+  const int nblk = batch / TBS;
+  int rc = ensure_partials(M, nblk);
+  if (rc != ECG_OK) return rc;
+  const long long nb = n / batch;
+  // Multipliers of the per-epoch affine permutation of the sample order (coprime with n).
+  static const unsigned long long primes[] = {1000003ull, 2000003ull, 3000017ull, 4000037ull, 5000011ull,
+                                              6000011ull, 7000003ull, 8000009ull, 9000011ull, 10000019ull};
+  for (int e = 0; e < epochs; e++) {
+    unsigned long long mult = 1ull;
+    for (int q = 0; q < 10; q++) {
+      const unsigned long long cand = primes[(q + e + seed) % 10];
+      if (gcd_u(cand, (unsigned long long)n) == 1ull) { mult = cand; break; }
+    }
+    unsigned long long z = seed * 0x9E3779B97F4A7C15ull + (unsigned long long)e * 0xD1B54A32D192ED03ull + 1ull;
+    z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ull;
+    const unsigned long long offs = (z ^ (z >> 29)) % (unsigned long long)n;
+    HIPCHK(hipMemsetAsync(M->loss_acc, 0, sizeof(double), M->stream));
+    for (long long b = 0; b < nb; b++) {
+      rc = launch_fwd_bwd(M, S, n, b * batch, batch, mult, offs, z | 1ull, 1.f / (float)batch, nblk);
+      if (rc != ECG_OK) return rc;
+      M->adam_t++;
+      const double bc1 = 1. - std::pow(0.9, (double)M->adam_t);
+      const double bc2 = 1. - std::pow(0.999, (double)M->adam_t);
+      const float lr_t = (float)(lr * std::sqrt(bc2) / bc1);
+      hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, M->stream, M->prm, M->grad,
+                         M->adam_m, M->adam_v, lr_t);
+    }
+    HIPCHK(hipGetLastError());
+    double acc = 0.;
+    HIPCHK(hipMemcpyAsync(&acc, M->loss_acc, sizeof(double), hipMemcpyDeviceToHost, M->stream));
+    HIPCHK(hipStreamSynchronize(M->stream));
+    if (epoch_loss) epoch_loss[e] = acc / (double)(nb * batch);
+  }
+  return mlp_pack(M);
+}
+
+int ecg_mlp_predict(ecg_mlp* M, ecg_samples* S, int64_t n, float* out) {
+  if (!M || !S || !out || n <= 0) return api_fail(ECG_E_ARG, "invalid prediction arguments");
+  const long long have = samples_count(S);
+  if (have < 0) return api_fail(ECG_E_HIP, "reading the sample counter failed");
+  if (n > have) return api_fail(ECG_E_ARG, "more records requested than the buffer holds");
+  float* d = nullptr;
+  HIPCHK(hipMalloc(&d, (size_t)n * sizeof(float)));
+  MlpDev md;
+  md.frag = M->frag;
+  md.bias = M->bias;
+  hipLaunchKernelGGL(k_mlp_predict, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, M->stream, md, S->buf, (long long)n, d);
+  hipError_t e1 = hipGetLastError();
+  hipError_t e2 = hipMemcpyAsync(out, d, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, M->stream);
+  hipError_t e3 = hipStreamSynchronize(M->stream);
+  hipFree(d);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return api_fail(ECG_E_HIP, "network inference failed");
+  return ECG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+// Learned Newton warm start of the infinite pressure relaxation (gfx950).
+//
+// The reference relaxation (RelaxationPInfinite::relaxation +
+// RelaxationP::NewtonRaphson, RelaxationPInfinite.cpp:43-98,
+// RelaxationP.cpp:43-93) starts Newton from the cold guess
+// p_cold = sum_k alpha_k p_k.  Here a 12 -> 64 -> 64 -> 1 GELU network
+// predicts the relaxed pressure instead; Newton then runs to the reference
+// tolerance from that guess, and the cold-start Newton stays the fallback.
+//
+// Scale-free inputs, per phase slot k (3 slots, unused slots zero), from the
+// pre-relaxation phase state (alpha_k, rho_k, p_k, e_k) through the EOS:
+//   x_k0 = alpha_k
+//   x_k1 = (p_k - p_cold) / s,          s = max_k |p_k - p_cold|
+//   x_k2 = (alpha_k / Z_k) / sum_j (alpha_j / Z_j),   Z_k = rho_k c_k^2 = gamma_k (p_k + pinf_k)
+//   x_k3 = s / (s + Z_k)
+// x_k2 are the weights of the linearised relaxation, so t_lin = sum_k x_k2 x_k1
+// is its normalised root; the network predicts the remainder:
+//   p_guess = p_cold + s * (t_lin + y).
+//
+// Inference: BF16 weights and activations, FP32 accumulation on
+// v_mfma_f32_16x16x16bf16_1k.  Cells are the N dimension of every product,
+// out^T[64 x 16 cells] = W^T[64 x K] in^T[K x 16 cells], so the accumulator
+// fragment of one layer (lane l: rows 4*(l/16)+j, column l%16) is, after bias,
+// GELU and BF16 rounding, exactly the B fragment of the next layer.
+#pragma once
+
+#include "ecg_physics.hpp"
+
+namespace ecg {
+
+constexpr int MLP_SLOTS = 3;
+constexpr int MLP_IN = 12;
+constexpr int MLP_H = 64;
+// FP32 parameter vector: W1[IN][H], b1[H], W2[H][H], b2[H], w3[H], b3.
+constexpr int MLP_OFF_W1 = 0;
+constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_IN * MLP_H;
+constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_H;
+constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H;
+constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_H;
+constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_H;
+constexpr int MLP_NPARAM = MLP_OFF_B3 + 1;
+// BF16 A fragments (64 lanes x 4 values): W1 tiles mt = 0..3, W2 tiles (mt, kt), w3 tiles kt.
+constexpr int MLP_FRAG_W1 = 0;
+constexpr int MLP_FRAG_W2 = 4;
+constexpr int MLP_FRAG_W3 = 20;
+constexpr int MLP_NFRAG = 24;
+// FP32 biases next to the fragments: b1[64], b2[64], b3.
+constexpr int MLP_NBIAS = 2 * MLP_H + 1;
+// Relaxation sample record: features[12], target t, cold-start iterations, s / |p*|, spare.
+constexpr int SAMPLE_FLOATS = 16;
+// Newton iterations allowed from the learned guess before the cold start takes over.
+constexpr int WARM_MAX_IT = 30;
+// The network is consulted only where the linearised residual at the cold
+// guess, sum_k alpha_k (p_k - p_cold) / Z_k, exceeds this (a decade below the
+// Newton tolerance 1e-10): elsewhere the cold guess converges at its first
+// evaluation and no guess can do better.
+constexpr double WARM_MIN_RESIDUAL = 1e-11;
+
+struct MlpDev {
+  const uint2* frag;     // MLP_NFRAG * 64 entries of 4 packed BF16
+  const float* bias;     // MLP_NBIAS
+};
+
+__device__ __forceinline__ float gelu_f(float x) {
+  const float u = 0.7978845608f * (x + 0.044715f * x * x * x);
+  const float th = 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(2.885390082f * u));
+  return 0.5f * x * (1.f + th);
+}
+// d gelu / dx of the same tanh form.
+__device__ __forceinline__ float gelu_grad_f(float x) {
+  const float u = 0.7978845608f * (x + 0.044715f * x * x * x);
+  const float th = 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(2.885390082f * u));
+  return 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * 0.7978845608f * (1.f + 0.134145f * x * x);
+}
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+__device__ __host__ __forceinline__ unsigned short bf16_bits(float f) {
+  unsigned int u = __builtin_bit_cast(unsigned int, f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ unsigned int pack_bf16x2(float a, float b) {
+  return (unsigned int)bf16_bits(a) | ((unsigned int)bf16_bits(b) << 16);
+}
+
+// Features of one cell before relaxation (double state); false when the cold
+// guess already satisfies the Newton test to first order (see WARM_MIN_RESIDUAL).
+template <int N>
+__device__ __forceinline__ bool relax_features(const Params& P, const Cell<N>& s, double& p_cold, double& spread,
+                                               float* x, float& t_lin) {
+  double pc = 0.;
+#pragma unroll
+  for (int k = 0; k < N; k++) pc += s.a[k] * s.p[k];
+  double sp = 0.;
+#pragma unroll
+  for (int k = 0; k < N; k++) sp = fmax(sp, fabs(s.p[k] - pc));
+  p_cold = pc;
+  spread = sp;
+#pragma unroll
+  for (int i = 0; i < MLP_IN; i++) x[i] = 0.f;
+  t_lin = 0.f;
+  if (!(sp > 0.)) return false;
+  double w[N], wsum = 0., Z[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    Z[k] = fmax(P.eos[k].g * (s.p[k] + P.eos[k].pinf), 1.e-300);
+    w[k] = s.a[k] / Z[k];
+    wsum += w[k];
+  }
+  double tl = 0., flin = 0.;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    flin += s.a[k] * (s.p[k] - pc) / Z[k];
+    const double dp = (s.p[k] - pc) / sp;
+    const double wk = w[k] / wsum;
+    x[4 * k + 0] = (float)s.a[k];
+    x[4 * k + 1] = (float)dp;
+    x[4 * k + 2] = (float)wk;
+    x[4 * k + 3] = (float)(sp / (sp + Z[k]));
+    tl += wk * dp;
+  }
+  t_lin = (float)tl;
+  return fabs(flin) > WARM_MIN_RESIDUAL;
+}
+
+typedef short v4i16 __attribute__((ext_vector_type(4)));
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4i16 as_v4i16(uint2 v) {
+  v4i16 r;
+  r[0] = (short)(v.x & 0xFFFFu); r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xFFFFu); r[3] = (short)(v.y >> 16);
+  return r;
+}
+
+// Bias + GELU on an accumulator fragment (rows 4*(l/16)+j of tile mt) -> BF16 B fragment.
+__device__ __forceinline__ v4i16 act_frag(const v4f32& acc, const float* __restrict__ bias, int mt, int lane) {
+  const int r0 = mt * 16 + 4 * (lane >> 4);
+  uint2 o;
+  o.x = pack_bf16x2(gelu_f(acc[0] + bias[r0 + 0]), gelu_f(acc[1] + bias[r0 + 1]));
+  o.y = pack_bf16x2(gelu_f(acc[2] + bias[r0 + 2]), gelu_f(acc[3] + bias[r0 + 3]));
+  return as_v4i16(o);
+}
+
+// Network output y for the lane's own cell.  Every lane of the wave must call
+// it (wave-uniform control flow); lanes without a cell pass zero features.
+// stage: 64 * 16 BF16 of LDS for the wave's features, then 64 floats of outputs.
+__device__ __forceinline__ float mlp_infer_wave(const MlpDev& M, const float* x, unsigned short* stage) {
+  const int lane = threadIdx.x & 63;
+  {
+    uint4 a, b;
+    a.x = pack_bf16x2(x[0], x[1]);  a.y = pack_bf16x2(x[2], x[3]);
+    a.z = pack_bf16x2(x[4], x[5]);  a.w = pack_bf16x2(x[6], x[7]);
+    b.x = pack_bf16x2(x[8], x[9]);  b.y = pack_bf16x2(x[10], x[11]);
+    b.z = 0u; b.w = 0u;
+    uint4* dst = reinterpret_cast<uint4*>(stage + lane * 16);
+    dst[0] = a;
+    dst[1] = b;
+  }
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  float* yout = reinterpret_cast<float*>(stage + 64 * 16);
+  const float* b1 = M.bias;
+  const float* b2 = M.bias + MLP_H;
+  const float b3 = M.bias[2 * MLP_H];
+  v4i16 w1[4], w3[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) w1[mt] = as_v4i16(M.frag[(MLP_FRAG_W1 + mt) * 64 + lane]);
+#pragma unroll
+  for (int kt = 0; kt < 4; kt++) w3[kt] = as_v4i16(M.frag[(MLP_FRAG_W3 + kt) * 64 + lane]);
+  for (int nt = 0; nt < 4; nt++) {
+    // B fragment of the inputs: features 4*(l/16)..+3 of cell nt*16 + l%16.
+    const uint2 xin = *reinterpret_cast<const uint2*>(stage + (nt * 16 + (lane & 15)) * 16 + 4 * (lane >> 4));
+    const v4i16 bx = as_v4i16(xin);
+    v4i16 h1[4], h2[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      v4f32 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[mt], bx, acc, 0, 0, 0);
+      h1[mt] = act_frag(acc, b1, mt, lane);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      v4f32 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < 4; kt++) {
+        const v4i16 a = as_v4i16(M.frag[(MLP_FRAG_W2 + mt * 4 + kt) * 64 + lane]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[kt], acc, 0, 0, 0);
+      }
+      h2[mt] = act_frag(acc, b2, mt, lane);
+    }
+    v4f32 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[kt], h2[kt], acc, 0, 0, 0);
+    // Row 0 of the output tile: lanes 0..15, element 0.
+    if (lane < 16) yout[nt * 16 + lane] = acc[0] + b3;
+  }
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  const float y = yout[lane];
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  return y;
+}
+
+}  // namespace ecg

@@ -73,18 +73,33 @@ struct Clock {
 
 struct Counters {
   unsigned long long newton_fail, newton_iters, relax_calls;
+  unsigned long long warm_calls;      // relaxations started from the learned guess
+  unsigned long long warm_fallbacks;  // of those, cold restarts after WARM_MAX_IT iterations
+  double resid_sum;                   // sum over relaxed cells of |f| at exit
 };
 
 // std::min / std::max tie semantics.
 template <typename R> __device__ __forceinline__ R dmin(R a, R b) { return (b < a) ? b : a; }
 template <typename R> __device__ __forceinline__ R dmax(R a, R b) { return (a < b) ? b : a; }
 
+// Division, square root and power of the face work.  double: the IEEE
+// operations (the FP64 path keeps the reference's correctly rounded results).
+// float: the hardware reciprocal, square root, log2 and exp2 (about 1 ulp
+// each), since the FP32 form's error is set by FP32 storage of the state, not
+// by the last bit of these operations.
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fpow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float fpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+
 // ---------------------------------------------------------------------------
 // EOS
 // ---------------------------------------------------------------------------
 template <typename R>
 __device__ __forceinline__ R eos_energy(const EosDev& q, R eps, R rho, R p) {
-  return (q.sg ? (p + R(q.gpinf)) : p) / R(q.gm1) / dmax<R>(rho, eps) + R(q.eref);
+  return fdiv(fdiv(q.sg ? (p + R(q.gpinf)) : p, R(q.gm1)), dmax<R>(rho, eps)) + R(q.eref);
 }
 template <typename R>
 __device__ __forceinline__ R eos_pressure(const EosDev& q, R rho, R e) {
@@ -93,11 +108,11 @@ __device__ __forceinline__ R eos_pressure(const EosDev& q, R rho, R e) {
 }
 template <typename R>
 __device__ __forceinline__ R eos_sound(const EosDev& q, R eps, R rho, R p) {
-  return sqrt(R(q.g) * (q.sg ? (p + R(q.pinf)) : p) / dmax<R>(rho, eps));
+  return fsqrt(fdiv(R(q.g) * (q.sg ? (p + R(q.pinf)) : p), dmax<R>(rho, eps)));
 }
 template <typename R>
 __device__ __forceinline__ R eos_p_isentropic(const EosDev& q, R eps, R p0, R rho0, R rhof) {
-  R base = pow(rhof / dmax<R>(rho0, eps), R(q.g));
+  R base = fpow(fdiv(rhof, dmax<R>(rho0, eps)), R(q.g));
   return q.sg ? (p0 + R(q.pinf)) * base - R(q.pinf) : p0 * base;
 }
 template <typename R>
@@ -131,11 +146,11 @@ __device__ __forceinline__ R limit(int kind, R s1, R s2) {
     if (prod > R(1e-9)) { out = dmin<R>(fabs(s1), fabs(s2)); if (s1 < R(0)) out = -out; }
   } else if (kind == LIM_VANLEER) {
     R sum = s1 + s2;
-    if ((fabs(sum) > R(1e-6)) && (prod > R(1e-6))) out = R(2.) * prod / sum;
+    if ((fabs(sum) > R(1e-6)) && (prod > R(1e-6))) out = fdiv(R(2.) * prod, sum);
   } else if (kind == LIM_VANALBADA) {
     R sum = s1 + s2;
     if ((fabs(s1) > R(1e-6)) && (fabs(s2) > R(1e-6)) && (fabs(sum) > R(1e-6)) && (prod > R(1e-6)))
-      out = prod * sum / (s1 * s1 + s2 * s2);
+      out = fdiv(prod * sum, s1 * s1 + s2 * s2);
   } else if (kind == LIM_SUPERBEE) {
     if (prod > R(1e-6)) out = dmax<R>(dmin<R>(R(2.) * fabs(s1), fabs(s2)), dmin<R>(fabs(s1), R(2.) * fabs(s2)));
     if (s1 < R(0.)) out = -out;
@@ -230,11 +245,11 @@ __device__ __forceinline__ void derive(const Params& P, Cell<N, R>& s, bool rebu
   R den = dmax<R>(rho, eps), em = R(0.), cf = R(0.);
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    R Y = s.a[k] * s.r[k] / den;
+    R Y = fdiv(s.a[k] * s.r[k], den);
     em += Y * s.e[k];
     cf += Y * s.c[k] * s.c[k];
   }
-  s.rho = rho; s.pm = pm; s.em = em; s.cF = sqrt(cf);
+  s.rho = rho; s.pm = pm; s.em = em; s.cF = fsqrt(cf);
 }
 
 // verifyAndCorrectPhase (PhaseUEq.cpp:355-370, PhaseEuler.cpp:395-399)
@@ -262,7 +277,7 @@ __device__ __forceinline__ void verify_correct(const Params& P, Cell<N, R>& s, i
 // ---------------------------------------------------------------------------
 template <typename R>
 __device__ __forceinline__ void cfl_bound(R& dtm, R s, R dx) {
-  if (fabs(s) > R(1.e-3)) dtm = dmin<R>(dtm, R(1.) * dx / fabs(s));
+  if (fabs(s) > R(1.e-3)) dtm = dmin<R>(dtm, fdiv(R(1.) * dx, fabs(s)));
 }
 
 // ModEuler::solveRiemannIntern
@@ -276,7 +291,7 @@ __device__ __forceinline__ void hllc_euler(const Cell<N, R>& Lc, const R* lu, co
   cfl_bound<R>(dtm, sL, lcfl);
   cfl_bound<R>(dtm, sR, lcfl);
   const R mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
-  R sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  R sM = fdiv(pR - pL + mL * uL - mR * uR, mL - mR);
   if (fabs(sM) < R(1.e-8)) sM = R(0.);
   R mass, m0, m1, m2, en;
   if (sL > R(0.)) {
@@ -288,8 +303,8 @@ __device__ __forceinline__ void hllc_euler(const Cell<N, R>& Lc, const R* lu, co
     const R mS = left ? mL : mR, uS = left ? uL : uR, pS = left ? pL : pR, sS = left ? sL : sR;
     const R ES = left ? EL : ER, vS = left ? vL : vR, wS = left ? wL : wR;
     const R pStar = mS * (sM - uS) + pS;
-    const R rhoStar = mS / (sS - sM);
-    const R Estar = ES + (sM - uS) * (sM + pS / mS);
+    const R rhoStar = fdiv(mS, sS - sM);
+    const R Estar = ES + (sM - uS) * (sM + fdiv(pS, mS));
     mass = rhoStar * sM; m0 = rhoStar * sM * sM + pStar; m1 = rhoStar * sM * vS; m2 = rhoStar * sM * wS;
     en = (rhoStar * Estar + pStar) * sM;
   }
@@ -309,7 +324,7 @@ __device__ __forceinline__ void hllc_ueq(const Params& P, const Cell<N, R>& Lc, 
   cfl_bound<R>(dtm, sL, lcfl);
   cfl_bound<R>(dtm, sR, lcfl);
   const R mL = rhoL * (sL - uL), mR = rhoR * (sR - uR);
-  R sM = (pR - pL + mL * uL - mR * uR) / (mL - mR);
+  R sM = fdiv(pR - pL + mL * uL - mR * uR, mL - mR);
   if (fabs(sM) < R(1.e-8)) sM = R(0.);
   if (sL >= R(0.) || sR <= R(0.)) {
     const bool left = (sL >= R(0.));
@@ -334,13 +349,13 @@ __device__ __forceinline__ void hllc_ueq(const Params& P, const Cell<N, R>& Lc, 
     const R* su = left ? lu : ru;
     const R uS = su[0], pS = left ? pL : pR, mS = left ? mL : mR, sS = left ? sL : sR;
     const R totalEnergy = S.em + R(0.5) * sqn3<R>(su);
-    const R rhoStar = mS / (sS - sM);
-    const R EStar = totalEnergy + (sM - uS) * (sM + pS / mS);
+    const R rhoStar = fdiv(mS, sS - sM);
+    const R EStar = totalEnergy + (sM - uS) * (sM + fdiv(pS, mS));
     const R pStar = mS * (sM - uS) + pS;
 #pragma unroll
     for (int k = 0; k < N; k++) {
       const R mk = S.r[k] * (sS - uS);
-      const R rks = mk / (sS - sM);
+      const R rks = fdiv(mk, sS - sM);
       const R pks = eos_p_isentropic<R>(P.eos[k], eps, S.p[k], S.r[k], rks);
       const R eks = eos_energy<R>(P.eos[k], eps, rks, pks);
       F.f.alpha[k] = S.a[k] * sM;
@@ -579,19 +594,24 @@ __device__ __forceinline__ void build_prim(const Params& P, Cons<N>& A, Cell<N>&
   s.E = A.energM / rhoMel;
 }
 
-// RelaxationPInfinite::relaxation + RelaxationP::NewtonRaphson from the initial
-// guess pStar (the reference's guess is sum alpha_k p_k).  Returns iterations
-// (0 when gated out).  FP64.
+// alphaNull gate of RelaxationPInfinite::relaxation (RelaxationPInfinite.cpp:51-55):
+// no relaxation when a phase fills the cell to within 1e-5.
 template <int N>
-__device__ __forceinline__ int relax_newton(const Params& P, Cell<N>& s, double pStar) {
+__device__ __forceinline__ bool relax_gated(const Params& P, const Cell<N>& s) {
   if (P.eps > 1.e-20) {
 #pragma unroll
     for (int k = 0; k < N; k++)
-      if (s.a[k] > (1. - 1.e-5)) return 0;
+      if (s.a[k] > (1. - 1.e-5)) return true;
   }
-  double ak[N], rk[N], pk[N], akS[N], rkS[N];
-#pragma unroll
-  for (int k = 0; k < N; k++) { ak[k] = s.a[k]; rk[k] = s.r[k]; pk[k] = s.p[k]; }
+  return false;
+}
+
+// Newton iterations of RelaxationP::NewtonRaphson (RelaxationP.cpp:43-70) from
+// pStar: evaluations until |f| <= 1e-10 or maxit.  f = sum_k alpha_k rho_k /
+// rho_k(p) - 1.  Returns the evaluation count; fres receives f at exit.
+template <int N>
+__device__ __forceinline__ int newton_core(const Params& P, const double* ak, const double* rk, const double* pk,
+                                           double& pStar, double* akS, double* rkS, int maxit, double& fres) {
   double f = 0., df = 1.;
   int it = 0;
   do {
@@ -610,7 +630,22 @@ __device__ __forceinline__ int relax_newton(const Params& P, Cell<N>& s, double 
       f += akS[k];
       df -= dalpha;
     }
-  } while (fabs(f) > 1e-10 && it < 100);
+  } while (fabs(f) > 1e-10 && it < maxit);
+  fres = f;
+  return it;
+}
+
+// RelaxationPInfinite::relaxation + RelaxationP::NewtonRaphson from the initial
+// guess pStar (the reference's guess is sum alpha_k p_k).  Returns iterations
+// (0 when gated out).  FP64.
+template <int N>
+__device__ __forceinline__ int relax_newton(const Params& P, Cell<N>& s, double pStar, double& fres) {
+  fres = 0.;
+  if (relax_gated<N>(P, s)) return 0;
+  double ak[N], rk[N], pk[N], akS[N], rkS[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) { ak[k] = s.a[k]; rk[k] = s.r[k]; pk[k] = s.p[k]; }
+  const int it = newton_core<N>(P, ak, rk, pk, pStar, akS, rkS, 100, fres);
   if (it < 100) {
 #pragma unroll
     for (int k = 0; k < N; k++) { s.a[k] = akS[k]; s.r[k] = rkS[k]; s.p[k] = pStar; }
@@ -641,6 +676,27 @@ __device__ __forceinline__ int relax_newton(const Params& P, Cell<N>& s, double 
     }
   }
   return it;
+}
+
+// Relaxation from a learned guess: Newton to the reference tolerance within
+// WARM_MAX_IT evaluations, else the reference procedure from the cold guess.
+// Caller has checked the gate.  Returns the total evaluation count.
+template <int N>
+__device__ __forceinline__ int relax_warm(const Params& P, Cell<N>& s, double guess, double p_cold, int maxit,
+                                          bool& fellback, double& fres) {
+  double ak[N], rk[N], pk[N], akS[N], rkS[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) { ak[k] = s.a[k]; rk[k] = s.r[k]; pk[k] = s.p[k]; }
+  double pStar = (guess == guess) ? guess : p_cold;
+  const int it = newton_core<N>(P, ak, rk, pk, pStar, akS, rkS, maxit, fres);
+  if (fabs(fres) <= 1e-10) {
+    fellback = false;
+#pragma unroll
+    for (int k = 0; k < N; k++) { s.a[k] = akS[k]; s.r[k] = rkS[k]; s.p[k] = pStar; }
+    return it;
+  }
+  fellback = true;
+  return it + relax_newton<N>(P, s, p_cold, fres);
 }
 
 template <int N>

@@ -104,12 +104,12 @@ __device__ __forceinline__ void slope_vars(const Cell<N, R>& L, const Cell<N, R>
   int v = 0;
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    if (MODEL != M_EULER) s[v++] = (Rr.a[k] - L.a[k]) / dist;
-    s[v++] = (Rr.r[k] - L.r[k]) / dist;
-    s[v++] = (Rr.p[k] - L.p[k]) / dist;
+    if (MODEL != M_EULER) s[v++] = fdiv(Rr.a[k] - L.a[k], dist);
+    s[v++] = fdiv(Rr.r[k] - L.r[k], dist);
+    s[v++] = fdiv(Rr.p[k] - L.p[k], dist);
   }
 #pragma unroll
-  for (int d = 0; d < D; d++) s[v++] = (Rr.u[d] - L.u[d]) / dist;
+  for (int d = 0; d < D; d++) s[v++] = fdiv(Rr.u[d] - L.u[d], dist);
 }
 
 // computeLocalSlopes + limitSlopes + extrapolate + verifyAndCorrectPhase on a
@@ -313,14 +313,46 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 
-template <int MODEL, int N, int D, bool O2, typename R>
+// Wave-aggregated append of relaxation samples (one atomic per wave).
+__device__ __forceinline__ void capture_samples(const SampleSink& K, bool cap, const float* feat, float target,
+                                                int iters, float rel_spread) {
+  const unsigned long long mask = __ballot(cap);
+  if (mask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned long long base = 0ull;
+  if (lane == leader) base = atomicAdd(K.seen, (unsigned long long)__popcll(mask));
+  const unsigned int lo = __shfl((unsigned int)(base & 0xFFFFFFFFull), leader, 64);
+  const unsigned int hi = __shfl((unsigned int)(base >> 32), leader, 64);
+  base = ((unsigned long long)hi << 32) | lo;
+  if (!cap) return;
+  const unsigned long long idx = base + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+  if (idx % (unsigned long long)K.stride != 0ull) return;
+  const unsigned long long slot = (unsigned long long)K.base + idx / (unsigned long long)K.stride;
+  if (slot >= (unsigned long long)K.cap) return;
+  float4* o = reinterpret_cast<float4*>(K.buf + slot * SAMPLE_FLOATS);
+  o[0] = make_float4(feat[0], feat[1], feat[2], feat[3]);
+  o[1] = make_float4(feat[4], feat[5], feat[6], feat[7]);
+  o[2] = make_float4(feat[8], feat[9], feat[10], feat[11]);
+  o[3] = make_float4(target, (float)iters, rel_spread, 0.f);
+}
+
+// LDS bytes one wave uses to stage the network's inputs and outputs.
+constexpr int MLP_STAGE_BYTES = 64 * 16 * 2 + 64 * 4;
+
+template <int MODEL, int N, int D, bool O2, typename R, bool LEARN>
 __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock* __restrict__ clk,
                                               Counters* __restrict__ cnt) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   R* lds = reinterpret_cast<R*>(lds_raw);
-  __shared__ double s_dt[4];
-  __shared__ unsigned long long s_c[3][4];
+  __shared__ double s_dt[4], s_res[4];
+  __shared__ unsigned long long s_c[5][4];
   if (clk->done) return;
   const double dt = clk->dt;
   int org[3];
@@ -334,7 +366,8 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock*
   const int64_t c = own[0] + (int64_t)own[1] * P.nx + (int64_t)own[2] * P.nx * P.ny;
 
   R dtm = R(1.e10);
-  unsigned long long n_it = 0, n_fail = 0, n_relax = 0;
+  unsigned long long n_it = 0, n_fail = 0, n_relax = 0, n_warm = 0, n_wfb = 0;
+  double resid = 0.;
   // Time-n cell: non-conservative terms and U^n (Cell::getPhase default type).
   Cell<N, R> ownR;
   R an[N], pn[N];
@@ -351,6 +384,13 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock*
   if (D >= 2) axis_pass<MODEL, N, D, O2, R, 1, FR_YI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
   if (D >= 3) axis_pass<MODEL, N, D, O2, R, 2, FR_ZI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
 
+  Cell<N> nw;
+  bool relax_now = false, spread_ok = false;
+  double p_cold = 0., spread = 0.;
+  float feat[MLP_IN];
+  float t_lin = 0.f;
+#pragma unroll
+  for (int q = 0; q < MLP_IN; q++) feat[q] = 0.f;
   if (own_ok) {
     // Boundary faces see the stage's own cell: W^n (first order / predictor) or W* (corrector).
     Cell<N, R> src;
@@ -389,17 +429,50 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock*
     const double scale = (O2 && L.stage == 1) ? 0.5 * dt : dt;
     Cons<N> A;
     time_evolution<MODEL, N, D, R>(P, acc, A, own64, scale);
-    Cell<N> nw;
     build_prim<MODEL, N>(P, A, nw);
+    if (MODEL != M_EULER && P.relax) {
+      relax_now = !relax_gated<N>(P, nw);
+      if (relax_now && (LEARN || L.sink.buf != nullptr))
+        spread_ok = relax_features<N>(P, nw, p_cold, spread, feat, t_lin);
+    }
+  }
+
+  // Learned warm start: the whole wave runs the network when any of its cells
+  // relaxes from distinct phase pressures.
+  float y = 0.f;
+  if constexpr (LEARN && MODEL != M_EULER) {
+    __syncthreads();   // the tile's LDS is free from here on
+    if (__ballot(spread_ok) != 0ull)
+      y = mlp_infer_wave(L.mlp, feat, reinterpret_cast<unsigned short*>(lds_raw + (tid >> 6) * MLP_STAGE_BYTES));
+  }
+
+  int it = 0;
+  if (own_ok && MODEL != M_EULER && P.relax) {
     // Relaxation: after the predictor (CellO2.cpp:128-134) and after the full step
     // (Run::advancingProcedure, Run.cpp:593).
-    if (MODEL != M_EULER && P.relax) {
-      const int it = relax_newton<N>(P, nw, relax_guess<N>(nw));
-      if (it > 0) { n_relax = 1; n_it = (unsigned long long)it; n_fail = (it >= 100) ? 1 : 0; }
-      correct_and_derive<MODEL, N>(P, nw);
-    } else if (O2 && L.stage == 1) {
-      correct_and_derive<MODEL, N>(P, nw);
+    double fres = 0.;
+    if (LEARN && spread_ok) {
+      bool fell = false;
+      const double guess = p_cold + spread * ((double)t_lin + (double)y);
+      it = relax_warm<N>(P, nw, guess, p_cold, WARM_MAX_IT, fell, fres);
+      n_warm = 1;
+      n_wfb = fell ? 1 : 0;
+    } else {
+      it = relax_newton<N>(P, nw, relax_guess<N>(nw), fres);
     }
+    if (it > 0) {
+      n_relax = 1; n_it = (unsigned long long)it; n_fail = (it >= 100) ? 1 : 0; resid = fabs(fres);
+    }
+  }
+  if (MODEL != M_EULER && L.sink.buf != nullptr) {
+    const bool cap = spread_ok && it > 0 && it < 100;
+    const float target = cap ? (float)((nw.p[0] - p_cold) / spread) : 0.f;
+    const float rel = cap ? (float)(spread / fmax(fabs(nw.p[0]), 1.e-300)) : 0.f;
+    capture_samples(L.sink, cap, feat, target, it, rel);
+  }
+  if (own_ok) {
+    if (MODEL != M_EULER && P.relax) correct_and_derive<MODEL, N>(P, nw);
+    else if (O2 && L.stage == 1) correct_and_derive<MODEL, N>(P, nw);
     store_prim<MODEL, N, D>(P, L.wout, c, nw);
   }
   // Block reductions: CFL minimum and relaxation counters.
@@ -407,18 +480,32 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock*
   n_it = wave_sum(n_it);
   n_fail = wave_sum(n_fail);
   n_relax = wave_sum(n_relax);
+  n_warm = wave_sum(n_warm);
+  n_wfb = wave_sum(n_wfb);
+  resid = wave_sum_d(resid);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) { s_dt[wv] = dtm64; s_c[0][wv] = n_it; s_c[1][wv] = n_fail; s_c[2][wv] = n_relax; }
+  if (lane == 0) {
+    s_dt[wv] = dtm64; s_res[wv] = resid;
+    s_c[0][wv] = n_it; s_c[1][wv] = n_fail; s_c[2][wv] = n_relax; s_c[3][wv] = n_warm; s_c[4][wv] = n_wfb;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double m = s_dt[0];
-    unsigned long long a = s_c[0][0], b = s_c[1][0], r = s_c[2][0];
-    for (int w = 1; w < 4; w++) { m = dmin<double>(m, s_dt[w]); a += s_c[0][w]; b += s_c[1][w]; r += s_c[2][w]; }
+    double m = s_dt[0], rs = s_res[0];
+    unsigned long long a = s_c[0][0], b = s_c[1][0], r = s_c[2][0], w = s_c[3][0], wf = s_c[4][0];
+    for (int q = 1; q < 4; q++) {
+      m = dmin<double>(m, s_dt[q]); rs += s_res[q];
+      a += s_c[0][q]; b += s_c[1][q]; r += s_c[2][q]; w += s_c[3][q]; wf += s_c[4][q];
+    }
     atomicMin(&clk->dtmax_bits, (unsigned long long)__double_as_longlong(m));
     if (r) {
       atomicAdd(&cnt->newton_iters, a);
       atomicAdd(&cnt->newton_fail, b);
       atomicAdd(&cnt->relax_calls, r);
+      atomicAdd(&cnt->resid_sum, rs);
+    }
+    if (w) {
+      atomicAdd(&cnt->warm_calls, w);
+      atomicAdd(&cnt->warm_fallbacks, wf);
     }
   }
 }
@@ -485,22 +572,36 @@ __global__ __launch_bounds__(256) void k_fill(Params P, const DomainDev* __restr
   store_prim<MODEL, N, D>(P, W, c, s);
 }
 
+template <int MODEL, int N, int D, bool O2, typename R, bool LEARN>
+void launch_step_v(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counters* cnt, hipStream_t s) {
+  using SZ = Sizes<MODEL, N, D>;
+  const int blocks = P.tiles[0] * P.tiles[1] * P.tiles[2];
+  size_t lds = sizeof(R) * SZ::lds_elems(O2);
+  if (LEARN && lds < (size_t)(4 * MLP_STAGE_BYTES)) lds = 4 * MLP_STAGE_BYTES;
+  auto k = k_step<MODEL, N, D, O2, R, LEARN>;
+  hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
+}
+
 template <int MODEL, int N, int D, typename R>
 void launch_step_t(const Params& P, const StepLaunch& Lv, Clock* clk, Counters* cnt, hipStream_t s) {
   StepLaunchT<R> L;
   L.wn = (const R*)Lv.wn; L.wsrc = (const R*)Lv.wsrc; L.wout = (R*)Lv.wout; L.stage = Lv.stage;
-  const int blocks = P.tiles[0] * P.tiles[1] * P.tiles[2];
-  using SZ = Sizes<MODEL, N, D>;
+  L.mlp = Lv.mlp;
+  L.sink = Lv.sink;
+  // The learned warm start exists in the FP32 form of the relaxing models only.
+  constexpr bool CAN_LEARN = (sizeof(R) == 4) && (MODEL != M_EULER);
+  const bool learn = CAN_LEARN && Lv.mlp.frag != nullptr;
   if (Lv.stage == 0) {
-    const size_t lds = sizeof(R) * SZ::lds_elems(false);
-    auto k = k_step<MODEL, N, D, false, R>;
-    hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
+    if constexpr (CAN_LEARN) {
+      if (learn) { launch_step_v<MODEL, N, D, false, R, true>(P, L, clk, cnt, s); return; }
+    }
+    launch_step_v<MODEL, N, D, false, R, false>(P, L, clk, cnt, s);
   } else {
-    const size_t lds = sizeof(R) * SZ::lds_elems(true);
-    auto k = k_step<MODEL, N, D, true, R>;
-    hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
+    if constexpr (CAN_LEARN) {
+      if (learn) { launch_step_v<MODEL, N, D, true, R, true>(P, L, clk, cnt, s); return; }
+    }
+    launch_step_v<MODEL, N, D, true, R, false>(P, L, clk, cnt, s);
   }
 }
 

@@ -1,0 +1,79 @@
+"""Training pipeline of the learned Newton warm start.
+
+Samples are the pre-relaxation phase states (as scale-free features) and the
+relaxed pressures of the plain FP64 solver's own runs; the network is trained
+on the GPU by the library's forward/backward/Adam kernels (ecg_mlp_train).
+Every step of the pipeline runs through the C ABI of ``libecogen_amd.so``.
+"""
+import time
+
+import numpy as np
+
+from . import Mlp, Samples, Solver
+from .case import kapila_bubbles_2d, ueq3_spheres_3d
+
+
+def default_sources(scale=1.0):
+    """Cases whose plain FP64 runs supply the samples: the 2D Kapila shock-bubble
+    case (2048^2) and the 3D three-phase inclusion case (256^3), both first order
+    with infinite pressure relaxation.  `scale` shrinks the meshes (tests)."""
+    n2 = max(64, int(round(2048 * scale)))
+    n3 = max(32, int(round(256 * scale)))
+    return [(kapila_bubbles_2d(n=n2, nbub=64, order=1), 20),
+            (ueq3_spheres_3d(n=n3, nsph=64), 12)]
+
+
+def collect_samples(sources, target=10_000_000, log=None):
+    """Run each source case in plain FP64 with a sample buffer attached.
+
+    A first pass counts the candidate cells of the run; the second keeps every
+    stride-th one so that each case contributes about target / len(sources)
+    records spread over the whole run.  Returns (Samples, info)."""
+    per = max(1, target // len(sources))
+    smp = Samples(per * len(sources))
+    info = []
+    t0 = time.perf_counter()
+    for case, steps in sources:
+        c = dict(case)
+        c["precision"] = "fp64"
+        probe = Samples(1)
+        s = Solver(c)
+        s.attach_samples(probe, 1)
+        s.run(max_iter=steps)
+        seen = probe.seen
+        probe.close()
+        stride = max(1, -(-seen // per))
+        s.close()
+        s = Solver(c)
+        s.attach_samples(smp, stride)
+        s.run(max_iter=steps)
+        s.close()
+        info.append(dict(case=case["name"], steps=steps, candidates=seen, stride=stride))
+        if log:
+            log(f"samples: {case['name']} {steps} steps, {seen} candidate cells, stride {stride}")
+    return smp, dict(records=len(smp), sources=info, seconds=round(time.perf_counter() - t0, 2))
+
+
+def train(smp, epochs=3, batch=4096, lr=2e-3, seed=7, log=None):
+    """Train a fresh network on the records of `smp`.  Returns (Mlp, info)."""
+    m = Mlp(seed=seed)
+    nchk = min(len(smp), 200_000)
+    loss0 = m.gradient(smp, nchk)[1]
+    t0 = time.perf_counter()
+    losses = m.train(smp, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    el = time.perf_counter() - t0
+    loss1 = m.gradient(smp, nchk)[1]
+    if log:
+        log(f"training: {epochs} epochs over {len(smp)} records in {el:.1f} s, loss {loss0:.3e} -> {loss1:.3e}")
+    return m, dict(epochs=epochs, batch=batch, lr=lr, loss_initial=loss0, loss_final=loss1,
+                   epoch_loss=[float(v) for v in losses], seconds=round(el, 2))
+
+
+def save_params(mlp, path):
+    np.save(path, mlp.get_params(), allow_pickle=False)
+
+
+def load_params(path, seed=1):
+    m = Mlp(seed=seed)
+    m.set_params(np.load(path, allow_pickle=False))
+    return m

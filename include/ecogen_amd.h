@@ -103,6 +103,9 @@ typedef struct {
   int64_t newton_fail;    /* relaxations that fell back to the interface pressure (cumulative) */
   int64_t newton_iters;   /* total Newton iterations (cumulative) */
   int64_t relax_calls;    /* relaxed cells (cumulative) */
+  int64_t warm_calls;     /* relaxations started from the learned guess (cumulative) */
+  int64_t warm_fallbacks; /* of those, restarted from the cold guess (cumulative) */
+  double resid_sum;       /* sum over relaxed cells of |f| when Newton stopped (cumulative) */
 } ecg_step_stats;
 
 typedef struct ecg_ctx ecg_ctx;
@@ -156,6 +159,67 @@ int ecg_kernel_times(ecg_ctx* ctx, double* ms_total, int64_t* launches, int nkin
  * buffers.  Profilers divide a kernel's FETCH_SIZE / WRITE_SIZE by what they
  * report for this launch to turn counter units into bytes. */
 int ecg_calibration_copy(ecg_ctx* ctx, int64_t nbytes);
+
+/* Field-wise comparison of two contexts on the same mesh and model (any
+ * precision forms): for f = 0..nvar-1 the persistent variables, f = nvar the
+ * mixture density sum_k alpha_k rho_k (Euler: rho); out[4f .. 4f+3] =
+ * sum|a-b|, sum|b|, max|a-b|, max|b| over the cells.  nfields = nvar + 1.
+ * A diagnostic of this library (error of the FP32 forms against the FP64 path);
+ * the reference has no counterpart. */
+int ecg_state_diff(ecg_ctx* a, ecg_ctx* b, double* out, int nfields);
+
+/* Learned Newton warm start ---------------------------------------------
+ * Extends RelaxationPInfinite::relaxation (src/Relaxations/RelaxationPInfinite.cpp:43-98),
+ * whose Newton (src/Relaxations/RelaxationP.cpp:43-70) starts from sum_k alpha_k p_k.
+ * The reference has no learned guess: these entry points have no counterpart there.
+ *
+ * A sample record is ECG_SAMPLE_FLOATS floats: 12 scale-free features of the
+ * pre-relaxation phase states (3 phase slots x [alpha_k, (p_k - p_cold)/s,
+ * linearised weight, s/(s + rho_k c_k^2)], s = max_k |p_k - p_cold|), the
+ * normalised relaxed pressure t = (p* - p_cold)/s, the cold-start Newton
+ * evaluations, s/|p*|, and one spare float. */
+#define ECG_MLP_INPUTS 12
+#define ECG_MLP_HIDDEN 64
+#define ECG_MLP_NPARAM 5057    /* W1[12][64], b1[64], W2[64][64], b2[64], w3[64], b3 */
+#define ECG_SAMPLE_FLOATS 16
+
+typedef struct ecg_samples ecg_samples;
+typedef struct ecg_mlp ecg_mlp;
+
+/* Device buffer of relaxation samples. */
+ecg_samples* ecg_samples_create(int64_t capacity);
+void ecg_samples_destroy(ecg_samples* s);
+int ecg_samples_reset(ecg_samples* s);
+int64_t ecg_samples_count(ecg_samples* s);   /* records held */
+int64_t ecg_samples_seen(ecg_samples* s);    /* candidate cells met while attached (every stride) */
+int ecg_samples_download(ecg_samples* s, float* host, int64_t n);
+int ecg_samples_upload(ecg_samples* s, const float* host, int64_t n);
+
+/* While attached, the context's steps append one record for every stride-th
+ * cell that relaxes (cold start, any precision form) from distinct phase
+ * pressures, after the records the buffer already holds (attaching restarts
+ * the candidate count; records beyond the capacity are dropped).  One buffer
+ * serves one attached context at a time.  NULL detaches. */
+int ecg_attach_samples(ecg_ctx* ctx, ecg_samples* s, int stride);
+
+/* 12 -> 64 -> 64 -> 1 GELU network predicting t - t_lin, t_lin = sum_k x_k2 x_k1.
+ * Parameters are FP32 on the device; inference packs them as BF16 for MFMA. */
+ecg_mlp* ecg_mlp_create(uint64_t seed);
+void ecg_mlp_destroy(ecg_mlp* m);
+int ecg_mlp_get_params(ecg_mlp* m, float* host);          /* ECG_MLP_NPARAM floats */
+int ecg_mlp_set_params(ecg_mlp* m, const float* host);    /* resets the optimiser */
+/* Mean squared error over the first n records (phase slots as stored) and its gradient. */
+int ecg_mlp_gradient(ecg_mlp* m, ecg_samples* s, int64_t n, float* grad, double* loss);
+/* Adam on every record of s, phase slots permuted at random per record; epoch_loss[e]
+ * receives the mean loss of epoch e. */
+int ecg_mlp_train(ecg_mlp* m, ecg_samples* s, int epochs, int batch, float lr, uint64_t seed,
+                  double* epoch_loss);
+/* BF16-MFMA inference of t (= t_lin + network) for the first n records. */
+int ecg_mlp_predict(ecg_mlp* m, ecg_samples* s, int64_t n, float* out);
+/* Use m's guess, p = p_cold + s (t_lin + y), in the relaxations of an
+ * ECG_FP32_LEARNED context; Newton still runs to |f| <= 1e-10 and restarts from
+ * the cold guess after 30 evaluations.  NULL returns to the cold start. */
+int ecg_set_warm_start(ecg_ctx* ctx, const ecg_mlp* m);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,144 @@
+"""CPU restatement of the learned Newton warm start (test infrastructure only).
+
+Used by the tests to check the HIP library's sample features, its training
+gradients and its BF16 inference; nothing in the product imports it.
+
+The reference has no learned component; what it fixes is the relaxation the
+network serves (RelaxationPInfinite.cpp:43-98, RelaxationP.cpp:43-70), whose
+root this module's targets come from (through the oracle's ``orc_relax_cell``).
+
+Pins:
+  * ``backward`` is checked against central finite differences of
+    ``forward`` in float64 (tests/test_learned_oracle.py);
+  * ``features``: x_k2 are the weights of the relaxation linearised at the
+    phase pressures, so sum_k x_k2 x_k1 must reproduce the root of the
+    linearised equation sum_k alpha_k (p - p_k) / Z_k = 0 (same test), and the
+    root of the full equation is checked against the oracle's Newton.
+"""
+import numpy as np
+
+N_IN = 12
+N_H = 64
+N_PARAM = N_IN * N_H + N_H + N_H * N_H + N_H + N_H + 1
+OFF_W1 = 0
+OFF_B1 = OFF_W1 + N_IN * N_H
+OFF_W2 = OFF_B1 + N_H
+OFF_B2 = OFF_W2 + N_H * N_H
+OFF_W3 = OFF_B2 + N_H
+OFF_B3 = OFF_W3 + N_H
+SAMPLE_FLOATS = 16
+K0 = 0.7978845608028654   # sqrt(2/pi)
+K1 = 0.044715
+
+
+def unpack(prm):
+    prm = np.asarray(prm, dtype=np.float64)
+    W1 = prm[OFF_W1:OFF_B1].reshape(N_IN, N_H)
+    b1 = prm[OFF_B1:OFF_W2]
+    W2 = prm[OFF_W2:OFF_B2].reshape(N_H, N_H)
+    b2 = prm[OFF_B2:OFF_W3]
+    w3 = prm[OFF_W3:OFF_B3]
+    b3 = prm[OFF_B3]
+    return W1, b1, W2, b2, w3, b3
+
+
+def gelu(z):
+    return 0.5 * z * (1. + np.tanh(K0 * (z + K1 * z ** 3)))
+
+
+def gelu_grad(z):
+    th = np.tanh(K0 * (z + K1 * z ** 3))
+    return 0.5 * (1. + th) + 0.5 * z * (1. - th * th) * K0 * (1. + 3. * K1 * z * z)
+
+
+def t_lin(x):
+    x = np.asarray(x, dtype=np.float64)
+    return sum(x[:, 4 * k + 2] * x[:, 4 * k + 1] for k in range(3))
+
+
+def forward(prm, x):
+    """Network output y (the remainder over t_lin) for features x (n, 12), float64."""
+    W1, b1, W2, b2, w3, b3 = unpack(prm)
+    x = np.asarray(x, dtype=np.float64)
+    z1 = x @ W1 + b1
+    h1 = gelu(z1)
+    z2 = h1 @ W2 + b2
+    h2 = gelu(z2)
+    return h2 @ w3 + b3
+
+
+def loss_and_grad(prm, rec):
+    """Mean of (y - (t - t_lin))^2 over the records and its gradient, float64."""
+    W1, b1, W2, b2, w3, b3 = unpack(prm)
+    rec = np.asarray(rec, dtype=np.float64)
+    x = rec[:, :N_IN]
+    r = rec[:, 12] - t_lin(x)
+    n = x.shape[0]
+    z1 = x @ W1 + b1
+    h1 = gelu(z1)
+    z2 = h1 @ W2 + b2
+    h2 = gelu(z2)
+    y = h2 @ w3 + b3
+    d = y - r
+    loss = float(np.mean(d * d))
+    dy = 2. * d / n
+    g = np.zeros(N_PARAM)
+    g[OFF_W3:OFF_B3] = h2.T @ dy
+    g[OFF_B3] = dy.sum()
+    dz2 = np.outer(dy, w3) * gelu_grad(z2)
+    g[OFF_W2:OFF_B2] = (h1.T @ dz2).ravel()
+    g[OFF_B2:OFF_W3] = dz2.sum(0)
+    dz1 = (dz2 @ W2.T) * gelu_grad(z1)
+    g[OFF_W1:OFF_B1] = (x.T @ dz1).ravel()
+    g[OFF_B1:OFF_W2] = dz1.sum(0)
+    return loss, g
+
+
+def bf16(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    a = np.asarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(prm, x):
+    """Inference as the MFMA path computes it: BF16 inputs, weights and hidden
+    activations, wide accumulation, FP32 biases; returns t_lin + y."""
+    W1, b1, W2, b2, w3, b3 = unpack(np.asarray(prm, dtype=np.float32))
+    x = np.asarray(x, dtype=np.float32)
+    xb = bf16(x)
+    h1 = bf16(gelu(xb @ bf16(W1) + b1))
+    h2 = bf16(gelu(h1 @ bf16(W2) + b2))
+    y = h2 @ bf16(w3) + b3
+    return t_lin(x.astype(np.float64)) + y
+
+
+def features(alpha, p, eos):
+    """Features, cold guess, spread and t_lin of one cell's phase states.
+
+    alpha, p: per phase (N = len(eos)); eos: dicts with gamma and pinf (0 for IG).
+    Returns (x[12], p_cold, s, t_lin, f_lin) or None when all phases share one
+    pressure; f_lin = sum_k alpha_k (p_k - p_cold) / Z_k is the linearised
+    residual at the cold guess (the library consults the network, and records
+    samples, only where |f_lin| > 1e-11)."""
+    n = len(eos)
+    pc = 0.
+    for k in range(n):
+        pc += alpha[k] * p[k]
+    s = max(abs(p[k] - pc) for k in range(n))
+    if not s > 0:
+        return None
+    Z = [max(eos[k]["gamma"] * (p[k] + eos[k].get("pinf", 0.)), 1e-300) for k in range(n)]
+    w = [alpha[k] / Z[k] for k in range(n)]
+    ws = sum(w)
+    x = np.zeros(N_IN)
+    tl = 0.
+    fl = 0.
+    for k in range(n):
+        fl += alpha[k] * (p[k] - pc) / Z[k]
+        dp = (p[k] - pc) / s
+        wk = w[k] / ws
+        x[4 * k:4 * k + 4] = (alpha[k], dp, wk, s / (s + Z[k]))
+        tl += wk * dp
+    return x, pc, s, tl, fl

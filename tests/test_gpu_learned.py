@@ -1,0 +1,168 @@
+"""Learned Newton warm start on the GPU: sample capture, training gradients and
+BF16-MFMA inference against the CPU checker, and what the trained network
+saves over the cold start (needs a GPU)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle.pyoracle as po
+from oracle import mlp_oracle as mo
+from ecogen_amd.case import IG_AIR, SG_EPOXY, SG_WATER, kapila_bubbles_2d
+
+DP = C.POINTER(C.c_double)
+EOS3 = [SG_WATER, SG_EPOXY, IG_AIR]
+
+
+def _uniform_case(model, eos, alpha, rho, p, nx=64):
+    return dict(name=f"uniform_{model}_{len(eos)}", nx=nx, ny=1, nz=1, lx=1.0, ly=1.0, lz=1.0, model=model,
+                eos=[dict(e) for e in eos], order=1, limiters=("none",) * 4, cfl=0.8, alpha_null=False,
+                relax_p=True, bc=("nonreflecting",) * 6,
+                domains=[dict(kind="entire", alpha=list(alpha), rho=list(rho), p=list(p), vel=[0., 0., 0.])])
+
+
+def _random_records(rng, n):
+    rec = np.zeros((n, mo.SAMPLE_FLOATS), dtype=np.float32)
+    for i in range(n):
+        k = int(rng.integers(2, 4))
+        a = rng.dirichlet(np.ones(k))
+        p = 10 ** rng.uniform(4.5, 9, size=k)
+        f = mo.features(a, p, EOS3[:k])
+        rec[i, :12] = f[0]
+        rec[i, 12] = f[3] + rng.normal(0, 0.05)
+    return rec
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,eos,alpha,rho,p", [
+    ("pueq", [SG_WATER, IG_AIR], (0.3, 0.7), (1000.0, 1.2), (3.e5, 1.e5)),
+    ("ueq", EOS3, (0.5, 0.3, 0.2), (1000.0, 1185.0, 1.2), (2.e5, 5.e6, 1.e5)),
+])
+def test_capture_matches_oracle_relaxation(model, eos, alpha, rho, p):
+    """A uniform state out of pressure equilibrium has no fluxes: every cell's
+    pre-relaxation state is the initial one, so each record must carry the
+    features of that state and the oracle relaxation's root and iteration count."""
+    from ecogen_amd import Samples, Solver
+    print(f"[learned] capture {model} {len(eos)} phases", flush=True)
+    case = _uniform_case(model, eos, alpha, rho, p)
+    s = Solver(case)
+    smp = Samples(1000)
+    s.attach_samples(smp, 1)
+    s.step(1.e-15)
+    rec = smp.download()
+    assert rec.shape[0] == case["nx"]
+    x, pc, sp, tl, fl = mo.features(alpha, p, eos)
+    assert abs(fl) > 1e-11
+    L = po.lib()
+    n = len(eos)
+    oe = (po.OrcEos * n)(*[po.make_eos(e) for e in eos])
+    a, r, pp, ps = np.array(alpha, float), np.array(rho, float), np.array(p, float), np.zeros(1)
+    it = L.orc_relax_cell(n, oe, 0.0, 0, a.ctypes.data_as(DP), r.ctypes.data_as(DP), pp.ctypes.data_as(DP),
+                          ps.ctypes.data_as(DP))
+    t = (ps[0] - pc) / sp
+    np.testing.assert_allclose(rec[:, :12], np.broadcast_to(x.astype(np.float32), (rec.shape[0], 12)),
+                               rtol=0, atol=2e-7)
+    np.testing.assert_allclose(rec[:, 12], t, rtol=2e-6, atol=2e-7)
+    assert np.all(rec[:, 13] == it)
+
+
+@pytest.mark.gpu
+def test_training_gradient_matches_oracle():
+    from ecogen_amd import Mlp, Samples
+    print("[learned] gradient", flush=True)
+    rng = np.random.default_rng(21)
+    rec = _random_records(rng, 1000)
+    prm = rng.normal(0, 0.2, mo.N_PARAM).astype(np.float32)
+    smp = Samples(rec.shape[0])
+    smp.upload(rec)
+    m = Mlp(seed=5)
+    m.set_params(prm)
+    g, loss = m.gradient(smp, rec.shape[0])
+    loss_o, g_o = mo.loss_and_grad(prm.astype(np.float64), rec.astype(np.float64))
+    assert abs(loss - loss_o) <= 1e-5 * loss_o
+    err = np.linalg.norm(g - g_o) / np.linalg.norm(g_o)
+    print(f"[learned] gradient relative L2 error {err:.2e}", flush=True)
+    assert err <= 1e-4
+    assert np.all(np.abs(g - g_o) <= 1e-4 * np.abs(g_o).max() + 1e-4 * np.abs(g_o))
+
+
+@pytest.mark.gpu
+def test_mfma_inference_matches_bf16_oracle():
+    from ecogen_amd import Mlp, Samples
+    print("[learned] inference", flush=True)
+    rng = np.random.default_rng(22)
+    rec = _random_records(rng, 3001)          # partial last wave
+    prm = rng.normal(0, 0.3, mo.N_PARAM).astype(np.float32)
+    smp = Samples(rec.shape[0])
+    smp.upload(rec)
+    m = Mlp(seed=6)
+    m.set_params(prm)
+    out = m.predict(smp)
+    ref = mo.forward_bf16(prm, rec[:, :12])
+    ref64 = mo.t_lin(rec[:, :12]) + mo.forward(prm.astype(np.float64), rec[:, :12].astype(np.float64))
+    d = np.abs(out - ref)
+    scale = 1.0 + np.abs(ref)
+    print(f"[learned] inference |gpu - bf16 oracle| max {np.max(d / scale):.2e} mean {np.mean(d / scale):.2e}; "
+          f"|bf16 oracle - fp64 network| mean {np.mean(np.abs(ref - ref64) / scale):.2e}", flush=True)
+    assert np.max(d / scale) <= 1e-2
+    assert np.mean(d / scale) <= 2e-4
+
+
+@pytest.mark.gpu
+def test_trained_warm_start_saves_newton_iterations():
+    """Train on samples of the plain FP64 solver, then compare the FP32 form's
+    Newton work with the cold guess, the linearised guess (network output zero)
+    and the learned guess on the same case."""
+    from ecogen_amd import Mlp, Samples, Solver
+    print("[learned] capture + train", flush=True)
+    case = kapila_bubbles_2d(n=128, nbub=8, order=1)
+    steps = 40
+    s = Solver(case)
+    smp = Samples(2_000_000)
+    s.attach_samples(smp, 1)
+    s.run(max_iter=steps)
+    n = len(smp)
+    print(f"[learned] {n} samples from {steps} FP64 steps", flush=True)
+    assert n > 2000
+    rec = smp.download()
+    assert np.all(np.isfinite(rec))
+    m = Mlp(seed=3)
+    nchk = min(n, 50000)
+    loss0 = m.gradient(smp, nchk)[1]
+    losses = m.train(smp, epochs=12, batch=1024, lr=2e-3, seed=4)
+    loss1 = m.gradient(smp, nchk)[1]
+    print(f"[learned] loss {loss0:.3e} -> {loss1:.3e} (epochs {', '.join(f'{v:.2e}' for v in losses)})", flush=True)
+    assert loss1 < 0.5 * loss0
+
+    def run(precision, mlp):
+        c = dict(case)
+        c["precision"] = precision
+        g = Solver(c)
+        if mlp is not None:
+            g.set_warm_start(mlp)
+        st = g.run(max_iter=steps)
+        return st, g.get_state()
+
+    zero = Mlp(seed=1)
+    zero.set_params(np.zeros(mo.N_PARAM, dtype=np.float32))
+    cold, s_cold = run("fp32", None)
+    lin, s_lin = run("fp32_learned", zero)
+    warm, s_warm = run("fp32_learned", m)
+    for nm, st in (("cold", cold), ("linearised", lin), ("learned", warm)):
+        print(f"[learned] {nm}: newton evaluations {st['newton_iters']} over {st['relax_calls']} relaxations "
+              f"({st['newton_iters'] / st['relax_calls']:.4f} each), warm-started {st['warm_calls']}, "
+              f"cold restarts {st['warm_fallbacks']}, mean |f| at exit {st['resid_sum'] / st['relax_calls']:.2e}",
+              flush=True)
+    saved = cold["newton_iters"] - warm["newton_iters"]
+    print(f"[learned] evaluations saved over the cold start: {saved} "
+          f"({saved / max(warm['warm_calls'], 1):.3f} per warm-started relaxation)", flush=True)
+    assert warm["warm_calls"] > 0
+    assert warm["newton_iters"] < cold["newton_iters"]
+    assert warm["warm_fallbacks"] <= 0.01 * warm["warm_calls"]
+    assert warm["newton_fail"] == 0
+
+    def mix(st):
+        return st[0] * st[1] + st[3] * st[4]
+    err = np.sum(np.abs(mix(s_warm) - mix(s_cold))) / np.sum(np.abs(mix(s_cold)))
+    print(f"[learned] mixture density rel L1 learned vs cold FP32 form: {err:.2e}", flush=True)
+    assert err <= 1e-5
