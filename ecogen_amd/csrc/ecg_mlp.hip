@@ -12,10 +12,10 @@
 
 namespace ecg {
 
-constexpr int TBS = 128;   // samples per training block
+constexpr int TBS = 64;    // samples per training block (one wave)
 constexpr int LDP = 65;    // padded row of the block's activation matrices
 constexpr int PRM_PAD = (MLP_NPARAM + 3) & ~3;
-constexpr size_t TRAIN_LDS_FLOATS = PRM_PAD + 3 * TBS * LDP + TBS * 13 + 2 * TBS;
+constexpr size_t TRAIN_LDS_FLOATS = PRM_PAD + 4 * TBS * LDP + TBS * 13 + 2 * TBS;
 
 // Orders of the three phase slots; training sees every order of every sample.
 __constant__ int k_slot_perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
@@ -26,9 +26,11 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
   return z ^ (z >> 31);
 }
 
-// Mean squared error of the remainder r = t - t_lin over `count` samples taken
+// Mean squared error of the clipped remainder r = clip(u) over `count` samples taken
 // as idx = (mult * (q0 + b) + offs) mod n; per-block parameter gradients of
-// inv_batch * sum (y - r)^2 and per-block loss sums.
+// inv_batch * sum (y - r)^2 and per-block loss sums.  Every activation matrix
+// of the block (64 samples x 64 units) lives in LDS; each thread owns one
+// sample's row and keeps one row of 64 accumulators in registers.
 __global__ __launch_bounds__(TBS) void k_mlp_fwd_bwd(const float* __restrict__ prm_g, const float* __restrict__ smp,
                                                      long long n, long long q0, long long count,
                                                      unsigned long long mult, unsigned long long offs,
@@ -36,10 +38,11 @@ __global__ __launch_bounds__(TBS) void k_mlp_fwd_bwd(const float* __restrict__ p
                                                      float* __restrict__ gpart, float* __restrict__ lpart) {
   extern __shared__ __align__(16) float sm[];
   float* prm = sm;
-  float* H1 = prm + PRM_PAD;
-  float* DZ2 = H1 + TBS * LDP;
-  float* G3 = DZ2 + TBS * LDP;
-  float* X = G3 + TBS * LDP;
+  float* Z1 = prm + PRM_PAD;      // pre-activations of layer 1, later dL/dz1
+  float* H1 = Z1 + TBS * LDP;     // gelu(z1)
+  float* Z2 = H1 + TBS * LDP;     // pre-activations of layer 2
+  float* DZ2 = Z2 + TBS * LDP;    // dL/dz2
+  float* X = DZ2 + TBS * LDP;
   float* DY = X + TBS * 13;
   float* LS = DY + TBS;
   const int t = threadIdx.x;
@@ -61,12 +64,11 @@ __global__ __launch_bounds__(TBS) void k_mlp_fwd_bwd(const float* __restrict__ p
 #pragma unroll
       for (int f = 0; f < 4; f++) x[4 * sl + f] = rec[4 * src + f];
     }
-    float tl = 0.f;
-#pragma unroll
-    for (int k = 0; k < MLP_SLOTS; k++) tl += x[4 * k + 2] * x[4 * k + 1];
-    r = rec[12] - tl;
+    r = fminf(fmaxf(rec[14], -MLP_TARGET_CLIP), MLP_TARGET_CLIP);
     msk = 1.f;
   }
+#pragma unroll
+  for (int i = 0; i < MLP_IN; i++) X[t * 13 + i] = x[i];
   __syncthreads();
   const float* W1 = prm + MLP_OFF_W1;
   const float* B1 = prm + MLP_OFF_B1;
@@ -74,85 +76,89 @@ __global__ __launch_bounds__(TBS) void k_mlp_fwd_bwd(const float* __restrict__ p
   const float* B2 = prm + MLP_OFF_B2;
   const float* W3 = prm + MLP_OFF_W3;
   const float B3 = prm[MLP_OFF_B3];
+  float* z1r = Z1 + t * LDP;
+  float* h1r = H1 + t * LDP;
+  float* z2r = Z2 + t * LDP;
+  float* d2r = DZ2 + t * LDP;
+  float acc[MLP_H];
 
-  float z1[MLP_H], h1[MLP_H], z2[MLP_H];
+  // Layer 1.
 #pragma unroll
-  for (int j = 0; j < MLP_H; j++) {
-    float a = B1[j];
+  for (int j = 0; j < MLP_H; j++) acc[j] = B1[j];
+#pragma unroll 1
+  for (int i = 0; i < MLP_IN; i++) {
+    const float xi = X[t * 13 + i];
 #pragma unroll
-    for (int i = 0; i < MLP_IN; i++) a += x[i] * W1[i * MLP_H + j];
-    z1[j] = a;
-    h1[j] = gelu_f(a);
+    for (int j = 0; j < MLP_H; j++) acc[j] += xi * W1[i * MLP_H + j];
   }
 #pragma unroll
-  for (int j = 0; j < MLP_H; j++) {
-    float a = B2[j];
+  for (int j = 0; j < MLP_H; j++) { z1r[j] = acc[j]; h1r[j] = gelu_f(acc[j]); }
+  // Layer 2.
 #pragma unroll
-    for (int i = 0; i < MLP_H; i++) a += h1[i] * W2[i * MLP_H + j];
-    z2[j] = a;
+  for (int j = 0; j < MLP_H; j++) acc[j] = B2[j];
+#pragma unroll 1
+  for (int i = 0; i < MLP_H; i++) {
+    const float hi = h1r[i];
+#pragma unroll
+    for (int j = 0; j < MLP_H; j++) acc[j] += hi * W2[i * MLP_H + j];
   }
+  // Output and the loss.
   float y = B3;
 #pragma unroll
-  for (int j = 0; j < MLP_H; j++) y += W3[j] * gelu_f(z2[j]);
+  for (int j = 0; j < MLP_H; j++) { z2r[j] = acc[j]; y += W3[j] * gelu_f(acc[j]); }
   const float d = y - r;
   const float loss = msk * d * d;
   const float dy = msk * 2.f * d * inv_batch;
-#pragma unroll
-  for (int j = 0; j < MLP_H; j++) {
-    H1[t * LDP + j] = h1[j];
-    G3[t * LDP + j] = dy * gelu_f(z2[j]);
-    const float g = dy * W3[j] * gelu_grad_f(z2[j]);
-    DZ2[t * LDP + j] = g;
-    z2[j] = g;
-  }
-#pragma unroll
-  for (int i = 0; i < MLP_H; i++) {
-    float a = 0.f;
-#pragma unroll
-    for (int j = 0; j < MLP_H; j++) a += W2[i * MLP_H + j] * z2[j];
-    z1[i] = a * gelu_grad_f(z1[i]);
-  }
   DY[t] = dy;
   LS[t] = loss;
 #pragma unroll
-  for (int i = 0; i < MLP_IN; i++) X[t * 13 + i] = x[i];
+  for (int j = 0; j < MLP_H; j++) d2r[j] = dy * W3[j] * gelu_grad_f(acc[j]);
+  // dL/dz1 = (W2 dz2) * gelu'(z1), row by row.
+#pragma unroll
+  for (int i = 0; i < MLP_H; i++) acc[i] = 0.f;
+#pragma unroll 1
+  for (int j = 0; j < MLP_H; j++) {
+    const float gj = d2r[j];
+#pragma unroll
+    for (int i = 0; i < MLP_H; i++) acc[i] += W2[i * MLP_H + j] * gj;
+  }
+#pragma unroll
+  for (int i = 0; i < MLP_H; i++) z1r[i] = acc[i] * gelu_grad_f(z1r[i]);
   __syncthreads();
 
+  // Block sums over the samples (fixed order); one bias / output weight per thread.
+  static_assert(TBS == MLP_H, "one thread per hidden unit in the bias sums");
   float* gp = gpart + (size_t)blockIdx.x * MLP_NPARAM;
+#pragma unroll 1
   for (int e = t; e < MLP_H * MLP_H; e += TBS) {
     const int i = e >> 6, j = e & 63;
     float a = 0.f;
-    for (int s = 0; s < TBS; s++) a += H1[s * LDP + i] * DZ2[s * LDP + j];
+    for (int s2 = 0; s2 < TBS; s2++) a += H1[s2 * LDP + i] * DZ2[s2 * LDP + j];
     gp[MLP_OFF_W2 + e] = a;
   }
-  if (t < MLP_H) {
-    float a = 0.f, c = 0.f;
-    for (int s = 0; s < TBS; s++) { a += DZ2[s * LDP + t]; c += G3[s * LDP + t]; }
-    gp[MLP_OFF_B2 + t] = a;
-    gp[MLP_OFF_W3 + t] = c;
-  } else if (t == MLP_H) {
-    float a = 0.f;
-    for (int s = 0; s < TBS; s++) a += DY[s];
-    gp[MLP_OFF_B3] = a;
-  } else if (t == MLP_H + 1) {
-    float a = 0.f;
-    for (int s = 0; s < TBS; s++) a += LS[s];
-    lpart[blockIdx.x] = a;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < MLP_H; i++) H1[t * LDP + i] = z1[i];   // dL/dz1
-  __syncthreads();
+#pragma unroll 1
   for (int e = t; e < MLP_IN * MLP_H; e += TBS) {
     const int i = e >> 6, j = e & 63;
     float a = 0.f;
-    for (int s = 0; s < TBS; s++) a += X[s * 13 + i] * H1[s * LDP + j];
+    for (int s2 = 0; s2 < TBS; s2++) a += X[s2 * 13 + i] * Z1[s2 * LDP + j];
     gp[MLP_OFF_W1 + e] = a;
   }
-  if (t < MLP_H) {
-    float a = 0.f;
-    for (int s = 0; s < TBS; s++) a += H1[s * LDP + t];
-    gp[MLP_OFF_B1 + t] = a;
+  {
+    float a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    for (int s2 = 0; s2 < TBS; s2++) {
+      a1 += Z1[s2 * LDP + t];
+      a2 += DZ2[s2 * LDP + t];
+      a3 += DY[s2] * gelu_f(Z2[s2 * LDP + t]);
+    }
+    gp[MLP_OFF_B1 + t] = a1;
+    gp[MLP_OFF_B2 + t] = a2;
+    gp[MLP_OFF_W3 + t] = a3;
+  }
+  if (t == 0) {
+    float a = 0.f, l = 0.f;
+    for (int s2 = 0; s2 < TBS; s2++) { a += DY[s2]; l += LS[s2]; }
+    gp[MLP_OFF_B3] = a;
+    lpart[blockIdx.x] = l;
   }
 }
 
@@ -215,24 +221,23 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ prm,
 }
 
 // Stand-alone inference over sample records with the device function the step
-// kernel fuses into the relaxation: out = t_lin + y.
+// kernel fuses into the relaxation: out = the network output y (the warm start
+// uses t_lin + sigma * y).
 __global__ __launch_bounds__(256) void k_mlp_predict(MlpDev M, const float* __restrict__ smp, long long n,
                                                      float* __restrict__ out) {
-  __shared__ __align__(16) unsigned char stage[4 * (64 * 16 * 2 + 64 * 4)];
+  __shared__ __align__(16) unsigned char stage[MLP_BLOCK_LDS];
+  __shared__ int s_cnt[4];
   const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
   float x[MLP_IN];
 #pragma unroll
   for (int i = 0; i < MLP_IN; i++) x[i] = 0.f;
-  float tl = 0.f;
   if (g < n) {
     const float* rec = smp + g * SAMPLE_FLOATS;
 #pragma unroll
     for (int i = 0; i < MLP_IN; i++) x[i] = rec[i];
-#pragma unroll
-    for (int k = 0; k < MLP_SLOTS; k++) tl += x[4 * k + 2] * x[4 * k + 1];
   }
-  const float y = mlp_infer_wave(M, x, reinterpret_cast<unsigned short*>(stage + (threadIdx.x >> 6) * (64 * 16 * 2 + 64 * 4)));
-  if (g < n) out[g] = tl + y;
+  const float y = mlp_block_infer(M, g < n, x, stage, s_cnt);
+  if (g < n) out[g] = y;
 }
 
 int api_fail(int code, const std::string& msg);

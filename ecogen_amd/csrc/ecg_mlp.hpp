@@ -14,8 +14,13 @@
 //   x_k2 = (alpha_k / Z_k) / sum_j (alpha_j / Z_j),   Z_k = rho_k c_k^2 = gamma_k (p_k + pinf_k)
 //   x_k3 = s / (s + Z_k)
 // x_k2 are the weights of the linearised relaxation, so t_lin = sum_k x_k2 x_k1
-// is its normalised root; the network predicts the remainder:
-//   p_guess = p_cold + s * (t_lin + y).
+// is its normalised root.  The remainder t - t_lin is second order in the
+// phase pressure spread; it is measured in units of
+//   sigma = max(|sum_k x_k2 (t_lin - x_k1)^2 s / Z_k|, 1e-9)
+// (computed in FP64) and the network predicts it in those units, clipped to
+// +-4 in training:  p_guess = p_cold + s * (t_lin + sigma * y).  Where the
+// relaxation is nearly linear sigma is tiny, so the network's BF16-limited
+// error cannot spoil the linearised guess.
 //
 // Inference: BF16 weights and activations, FP32 accumulation on
 // v_mfma_f32_16x16x16bf16_1k.  Cells are the N dimension of every product,
@@ -46,7 +51,9 @@ constexpr int MLP_FRAG_W3 = 20;
 constexpr int MLP_NFRAG = 24;
 // FP32 biases next to the fragments: b1[64], b2[64], b3.
 constexpr int MLP_NBIAS = 2 * MLP_H + 1;
-// Relaxation sample record: features[12], target t, cold-start iterations, s / |p*|, spare.
+// Relaxation sample record: features[12], t = (p* - p_cold)/s, cold-start
+// iterations, u = (t - t_lin)/sigma (FP64-computed training target), sigma.
+constexpr float MLP_TARGET_CLIP = 4.f;
 constexpr int SAMPLE_FLOATS = 16;
 // Newton iterations allowed from the learned guess before the cold start takes over.
 constexpr int WARM_MAX_IT = 30;
@@ -87,7 +94,7 @@ __device__ __forceinline__ unsigned int pack_bf16x2(float a, float b) {
 // guess already satisfies the Newton test to first order (see WARM_MIN_RESIDUAL).
 template <int N>
 __device__ __forceinline__ bool relax_features(const Params& P, const Cell<N>& s, double& p_cold, double& spread,
-                                               float* x, float& t_lin) {
+                                               float* x, float& t_lin, double& t_lin_d, double& sigma) {
   double pc = 0.;
 #pragma unroll
   for (int k = 0; k < N; k++) pc += s.a[k] * s.p[k];
@@ -99,18 +106,21 @@ __device__ __forceinline__ bool relax_features(const Params& P, const Cell<N>& s
 #pragma unroll
   for (int i = 0; i < MLP_IN; i++) x[i] = 0.f;
   t_lin = 0.f;
+  t_lin_d = 0.;
+  sigma = 1e-9;
   if (!(sp > 0.)) return false;
-  double w[N], wsum = 0., Z[N];
+  double w[N], wsum = 0., Z[N], flin = 0.;
 #pragma unroll
   for (int k = 0; k < N; k++) {
     Z[k] = fmax(P.eos[k].g * (s.p[k] + P.eos[k].pinf), 1.e-300);
     w[k] = s.a[k] / Z[k];
     wsum += w[k];
+    flin += w[k] * (s.p[k] - pc);
   }
-  double tl = 0., flin = 0.;
+  if (!(fabs(flin) > WARM_MIN_RESIDUAL)) return false;
+  double tl = 0.;
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    flin += s.a[k] * (s.p[k] - pc) / Z[k];
     const double dp = (s.p[k] - pc) / sp;
     const double wk = w[k] / wsum;
     x[4 * k + 0] = (float)s.a[k];
@@ -119,8 +129,16 @@ __device__ __forceinline__ bool relax_features(const Params& P, const Cell<N>& s
     x[4 * k + 3] = (float)(sp / (sp + Z[k]));
     tl += wk * dp;
   }
+  double r2 = 0.;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const double d = tl - (s.p[k] - pc) / sp;
+    r2 += (w[k] / wsum) * d * d * (sp / Z[k]);
+  }
+  sigma = fmax(fabs(r2), 1e-9);
   t_lin = (float)tl;
-  return fabs(flin) > WARM_MIN_RESIDUAL;
+  t_lin_d = tl;
+  return true;
 }
 
 typedef short v4i16 __attribute__((ext_vector_type(4)));
@@ -142,61 +160,87 @@ __device__ __forceinline__ v4i16 act_frag(const v4f32& acc, const float* __restr
   return as_v4i16(o);
 }
 
-// Network output y for the lane's own cell.  Every lane of the wave must call
-// it (wave-uniform control flow); lanes without a cell pass zero features.
-// stage: 64 * 16 BF16 of LDS for the wave's features, then 64 floats of outputs.
-__device__ __forceinline__ float mlp_infer_wave(const MlpDev& M, const float* x, unsigned short* stage) {
+// LDS a 256-thread block needs for mlp_block_infer: 256 rows of 16 BF16
+// features, then 256 FP32 outputs.
+constexpr int MLP_BLOCK_LDS = 256 * 16 * 2 + 256 * 4;
+
+// Network output for one tile of 16 staged rows (rows nt*16 .. nt*16+15);
+// one wave, wave-uniform control flow.
+__device__ __forceinline__ void mlp_infer_tile(const MlpDev& M, const unsigned short* fstage, float* ystage, int nt) {
   const int lane = threadIdx.x & 63;
-  {
+  // B fragment of the inputs: features 4*(l/16)..+3 of row nt*16 + l%16.
+  const uint2 xin = *reinterpret_cast<const uint2*>(fstage + (nt * 16 + (lane & 15)) * 16 + 4 * (lane >> 4));
+  const v4i16 bx = as_v4i16(xin);
+  const float* b1 = M.bias;
+  const float* b2 = M.bias + MLP_H;
+  v4i16 h1[4], h2[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    v4f32 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(as_v4i16(M.frag[(MLP_FRAG_W1 + mt) * 64 + lane]), bx, acc, 0, 0, 0);
+    h1[mt] = act_frag(acc, b1, mt, lane);
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    v4f32 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) {
+      const v4i16 a = as_v4i16(M.frag[(MLP_FRAG_W2 + mt * 4 + kt) * 64 + lane]);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[kt], acc, 0, 0, 0);
+    }
+    h2[mt] = act_frag(acc, b2, mt, lane);
+  }
+  v4f32 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kt = 0; kt < 4; kt++)
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(as_v4i16(M.frag[(MLP_FRAG_W3 + kt) * 64 + lane]), h2[kt], acc, 0, 0,
+                                                    0);
+  // Row 0 of the output tile: lanes 0..15, element 0.
+  if (lane < 16) ystage[nt * 16 + lane] = acc[0] + M.bias[2 * MLP_H];
+}
+
+// Network output y for every thread of a 256-thread block whose `need` is set
+// (others get 0).  The block's candidates are compacted into LDS rows, and the
+// 16-row tiles they fill are shared out over the 4 waves, so the cost follows
+// the number of candidates, not the number of cells.  Must be called by every
+// thread of the block (it synchronises); lds: MLP_BLOCK_LDS bytes.
+__device__ __forceinline__ float mlp_block_infer(const MlpDev& M, bool need, const float* x, unsigned char* lds,
+                                                 int* s_cnt) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  unsigned short* fstage = reinterpret_cast<unsigned short*>(lds);
+  float* ystage = reinterpret_cast<float*>(lds + 256 * 16 * 2);
+  const unsigned long long m = __ballot(need);
+  if (lane == 0) s_cnt[wv] = __popcll(m);
+  __syncthreads();
+  int base = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    const int cw = s_cnt[w];
+    if (w < wv) base += cw;
+    total += cw;
+  }
+  const int rank = base + __popcll(m & ((1ull << lane) - 1ull));
+  if (need) {
     uint4 a, b;
     a.x = pack_bf16x2(x[0], x[1]);  a.y = pack_bf16x2(x[2], x[3]);
     a.z = pack_bf16x2(x[4], x[5]);  a.w = pack_bf16x2(x[6], x[7]);
     b.x = pack_bf16x2(x[8], x[9]);  b.y = pack_bf16x2(x[10], x[11]);
     b.z = 0u; b.w = 0u;
-    uint4* dst = reinterpret_cast<uint4*>(stage + lane * 16);
+    uint4* dst = reinterpret_cast<uint4*>(fstage + rank * 16);
     dst[0] = a;
     dst[1] = b;
   }
-  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  float* yout = reinterpret_cast<float*>(stage + 64 * 16);
-  const float* b1 = M.bias;
-  const float* b2 = M.bias + MLP_H;
-  const float b3 = M.bias[2 * MLP_H];
-  v4i16 w1[4], w3[4];
-#pragma unroll
-  for (int mt = 0; mt < 4; mt++) w1[mt] = as_v4i16(M.frag[(MLP_FRAG_W1 + mt) * 64 + lane]);
-#pragma unroll
-  for (int kt = 0; kt < 4; kt++) w3[kt] = as_v4i16(M.frag[(MLP_FRAG_W3 + kt) * 64 + lane]);
-  for (int nt = 0; nt < 4; nt++) {
-    // B fragment of the inputs: features 4*(l/16)..+3 of cell nt*16 + l%16.
-    const uint2 xin = *reinterpret_cast<const uint2*>(stage + (nt * 16 + (lane & 15)) * 16 + 4 * (lane >> 4));
-    const v4i16 bx = as_v4i16(xin);
-    v4i16 h1[4], h2[4];
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-      v4f32 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[mt], bx, acc, 0, 0, 0);
-      h1[mt] = act_frag(acc, b1, mt, lane);
-    }
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-      v4f32 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kt = 0; kt < 4; kt++) {
-        const v4i16 a = as_v4i16(M.frag[(MLP_FRAG_W2 + mt * 4 + kt) * 64 + lane]);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[kt], acc, 0, 0, 0);
-      }
-      h2[mt] = act_frag(acc, b2, mt, lane);
-    }
-    v4f32 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kt = 0; kt < 4; kt++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[kt], h2[kt], acc, 0, 0, 0);
-    // Row 0 of the output tile: lanes 0..15, element 0.
-    if (lane < 16) yout[nt * 16 + lane] = acc[0] + b3;
+  __syncthreads();
+  const int ntiles = (total + 15) >> 4;
+  // Keeps the weight fragments in L1/L2 rather than in registers for the whole kernel.
+#pragma unroll 1
+  for (int nt = wv; nt < ntiles; nt += 4) {
+    asm volatile("" ::: "memory");
+    mlp_infer_tile(M, fstage, ystage, nt);
   }
-  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  const float y = yout[lane];
-  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __syncthreads();
+  const float y = need ? ystage[rank] : 0.f;
+  __syncthreads();
   return y;
 }
 

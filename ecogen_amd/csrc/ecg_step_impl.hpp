@@ -36,18 +36,33 @@ template <int D, int A> struct TileAxis {
   static constexpr int NFACE = (T + 1) * LINES;
 };
 
+// LDS bytes one batch of an axis pass may stage (slopes + fluxes); larger
+// passes are split into batches of lines so that more blocks fit on a CU.
+constexpr int LDS_BATCH_BYTES = 32768;
+
 // Variables kept per face slope: phase (alpha, rho, p) x N (Euler: rho, p), velocity on the active axes.
 template <int MODEL, int N, int D> struct Sizes {
   static constexpr int PV = (MODEL == M_EULER) ? 2 : 3 * N;
   static constexpr int NS = PV + D;
   // flux: alpha, mass, energ per phase (Euler: mass), momentum (D), energy, sM, uStar
   static constexpr int NF = ((MODEL == M_EULER) ? 1 : 3 * N) + D + 1 + 2;
-  template <int A> static constexpr int lds_axis(bool o2) {
-    return (o2 ? TileAxis<D, A>::NSLOPE * NS : 0) + TileAxis<D, A>::NFACE * NF;
+  // Elements staged per line of axis A.
+  template <int A> static constexpr int per_line(bool o2) {
+    return (o2 ? (TileAxis<D, A>::T + 3) * NS : 0) + (TileAxis<D, A>::T + 1) * NF;
+  }
+  // Batches of lines of axis A for elements of esize bytes.
+  template <int A> static constexpr int batches(bool o2, int esize) {
+    int nb = 1;
+    while (nb < TileAxis<D, A>::LINES && per_line<A>(o2) * (TileAxis<D, A>::LINES / nb) * esize > LDS_BATCH_BYTES)
+      nb *= 2;
+    return nb;
+  }
+  template <int A> static constexpr int lds_axis(bool o2, int esize) {
+    return per_line<A>(o2) * (TileAxis<D, A>::LINES / batches<A>(o2, esize));
   }
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
-  static constexpr int lds_elems(bool o2) {
-    return cmax(lds_axis<0>(o2), cmax(D >= 2 ? lds_axis<1>(o2) : 0, D >= 3 ? lds_axis<2>(o2) : 0));
+  static constexpr int lds_elems(bool o2, int esize) {
+    return cmax(lds_axis<0>(o2, esize), cmax(D >= 2 ? lds_axis<1>(o2, esize) : 0, D >= 3 ? lds_axis<2>(o2, esize) : 0));
   }
 };
 
@@ -205,7 +220,8 @@ __device__ __forceinline__ void tile_origin(const Params& P, int& i0, int& j0, i
   k0 = tz * Tile<D>::Z;
 }
 
-// One axis pass (steps A, B, C above) for axis AX with face frame FR.
+// One axis pass (steps A, B, C above) for axis AX with face frame FR, in
+// batches of lines when the staging would exceed LDS_BATCH_BYTES.
 template <int MODEL, int N, int D, bool O2, typename R, int AX, int FR>
 __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>& L, R* lds, const int org[3],
                                           const int own[3], bool own_ok, const R* an, const R* pn, R& dtm,
@@ -213,6 +229,8 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
   using TA = TileAxis<D, AX>;
   using SZ = Sizes<MODEL, N, D>;
   constexpr int T = TA::T;
+  constexpr int NB = SZ::template batches<AX>(O2, (int)sizeof(R));
+  constexpr int LB = TA::LINES / NB;              // lines per batch
   constexpr int B = (AX == 0) ? 1 : 0;           // first line axis
   constexpr int C = (AX == 2) ? 1 : 2;           // second line axis
   constexpr int TB = (B == 0) ? Tile<D>::X : Tile<D>::Y;
@@ -221,77 +239,83 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
   const int na = n3[AX];
   const int64_t sa = st3[AX];
   R* s_slope = lds;
-  R* s_flux = lds + (O2 ? TA::NSLOPE * SZ::NS : 0);
+  R* s_flux = lds + (O2 ? (T + 3) * LB * SZ::NS : 0);
   const int tid = threadIdx.x;
+  const int lown = (own[B] - org[B]) + TB * (own[C] - org[C]);   // line of the own cell
 
-  // A. face slopes s_g, g = -2 .. T (interior faces only)
-  if (O2) {
-    for (int q = tid; q < TA::NSLOPE; q += 256) {
-      const int g = q % (T + 3) - 2;
-      const int l = q / (T + 3);
+#pragma unroll 1
+  for (int bt = 0; bt < NB; bt++) {
+    const int l0 = bt * LB;
+    // A. face slopes s_g, g = -2 .. T (interior faces only)
+    if (O2) {
+      for (int q = tid; q < (T + 3) * LB; q += 256) {
+        const int g = q % (T + 3) - 2;
+        const int l = l0 + q / (T + 3);
+        const int cb = org[B] + l % TB, cc = org[C] + l / TB;
+        const int m = org[AX] + g;
+        R* o = s_slope + q * SZ::NS;
+        if (cb < n3[B] && cc < n3[C] && m >= 0 && m <= na - 2) {
+          const int64_t cell = (int64_t)m * sa + (int64_t)cb * st3[B] + (int64_t)cc * st3[C];
+          Cell<N, R> cl, cr;
+          load_prim<MODEL, N, D>(P, L.wsrc, cell, cl);
+          load_prim<MODEL, N, D>(P, L.wsrc, cell + sa, cr);
+          slope_vars<MODEL, N, D, R>(cl, cr, R(P.dcc[AX][m]), o);
+        }
+      }
+      __syncthreads();
+    }
+
+    // B. interior face fluxes f = -1 .. T-1
+    for (int q = tid; q < (T + 1) * LB; q += 256) {
+      const int f = q % (T + 1) - 1;
+      const int lb = q / (T + 1);
+      const int l = l0 + lb;
       const int cb = org[B] + l % TB, cc = org[C] + l / TB;
-      const int m = org[AX] + g;
-      R* o = s_slope + q * SZ::NS;
+      const int m = org[AX] + f;
       if (cb < n3[B] && cc < n3[C] && m >= 0 && m <= na - 2) {
         const int64_t cell = (int64_t)m * sa + (int64_t)cb * st3[B] + (int64_t)cc * st3[C];
-        Cell<N, R> cl, cr;
-        load_prim<MODEL, N, D>(P, L.wsrc, cell, cl);
-        load_prim<MODEL, N, D>(P, L.wsrc, cell + sa, cr);
-        slope_vars<MODEL, N, D, R>(cl, cr, R(P.dcc[AX][m]), o);
+        Cell<N, R> bl, br;
+        load_prim<MODEL, N, D>(P, L.wsrc, cell, bl);
+        load_prim<MODEL, N, D>(P, L.wsrc, cell + sa, br);
+        if (O2) {
+          const R aL = (MODEL == M_EULER) ? R(0.) : bl.a[0];
+          const R aR = (MODEL == M_EULER) ? R(0.) : br.a[0];
+          const R* s1 = s_slope + (lb * (T + 3) + f + 2) * SZ::NS;
+          const bool hasm = (m >= 1), hasp = (m + 1 <= na - 2);
+          const R aLL = (MODEL != M_EULER && hasm) ? alpha0_at<R>(L.wn, cell - sa) : R(0.);
+          const R aRR = (MODEL != M_EULER && hasp) ? alpha0_at<R>(L.wn, cell + 2 * sa) : R(0.);
+          extrapolate<MODEL, N, D, R>(P, bl, s1, s1 - SZ::NS, hasm, aLL, aL, aR, R(P.dfl[AX][m]), false);
+          extrapolate<MODEL, N, D, R>(P, br, s1, s1 + SZ::NS, hasp, aRR, aR, aL, R(P.dfr[AX][m]), true);
+          derive<MODEL, N, R>(P, bl, MODEL == M_EULER);
+          derive<MODEL, N, R>(P, br, MODEL == M_EULER);
+        } else {
+          derive<MODEL, N, R>(P, bl, false);
+          derive<MODEL, N, R>(P, br, false);
+        }
+        Flux<N, R> F;
+        face_flux<MODEL, N, FR, R>(P, bl, bl.E, br, br.E, dtm, F);
+        flux_to_lds<MODEL, N, D, R>(F, s_flux + q * SZ::NF);
+      }
+    }
+    __syncthreads();
+
+    // C. gather: add the lower face, subtract the upper face
+    if (own_ok && lown / LB == bt) {
+      const int t = own[AX] - org[AX];
+      const int lb = lown - l0;
+      const R coef = R(P.coef[AX]);
+      Flux<N, R> F;
+      if (own[AX] > 0) {
+        flux_from_lds<MODEL, N, D, R>(s_flux + (lb * (T + 1) + t) * SZ::NF, F);
+        add_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
+      }
+      if (own[AX] < na - 1) {
+        flux_from_lds<MODEL, N, D, R>(s_flux + (lb * (T + 1) + t + 1) * SZ::NF, F);
+        sub_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
       }
     }
     __syncthreads();
   }
-
-  // B. interior face fluxes f = -1 .. T-1
-  for (int q = tid; q < TA::NFACE; q += 256) {
-    const int f = q % (T + 1) - 1;
-    const int l = q / (T + 1);
-    const int cb = org[B] + l % TB, cc = org[C] + l / TB;
-    const int m = org[AX] + f;
-    if (cb < n3[B] && cc < n3[C] && m >= 0 && m <= na - 2) {
-      const int64_t cell = (int64_t)m * sa + (int64_t)cb * st3[B] + (int64_t)cc * st3[C];
-      Cell<N, R> bl, br;
-      load_prim<MODEL, N, D>(P, L.wsrc, cell, bl);
-      load_prim<MODEL, N, D>(P, L.wsrc, cell + sa, br);
-      if (O2) {
-        const R aL = (MODEL == M_EULER) ? R(0.) : bl.a[0];
-        const R aR = (MODEL == M_EULER) ? R(0.) : br.a[0];
-        const R* s1 = s_slope + (l * (T + 3) + f + 2) * SZ::NS;
-        const bool hasm = (m >= 1), hasp = (m + 1 <= na - 2);
-        const R aLL = (MODEL != M_EULER && hasm) ? alpha0_at<R>(L.wn, cell - sa) : R(0.);
-        const R aRR = (MODEL != M_EULER && hasp) ? alpha0_at<R>(L.wn, cell + 2 * sa) : R(0.);
-        extrapolate<MODEL, N, D, R>(P, bl, s1, s1 - SZ::NS, hasm, aLL, aL, aR, R(P.dfl[AX][m]), false);
-        extrapolate<MODEL, N, D, R>(P, br, s1, s1 + SZ::NS, hasp, aRR, aR, aL, R(P.dfr[AX][m]), true);
-        derive<MODEL, N, R>(P, bl, MODEL == M_EULER);
-        derive<MODEL, N, R>(P, br, MODEL == M_EULER);
-      } else {
-        derive<MODEL, N, R>(P, bl, false);
-        derive<MODEL, N, R>(P, br, false);
-      }
-      Flux<N, R> F;
-      face_flux<MODEL, N, FR, R>(P, bl, bl.E, br, br.E, dtm, F);
-      flux_to_lds<MODEL, N, D, R>(F, s_flux + q * SZ::NF);
-    }
-  }
-  __syncthreads();
-
-  // C. gather: add the lower face, subtract the upper face
-  if (own_ok) {
-    const int t = own[AX] - org[AX];
-    const int l = (own[B] - org[B]) + TB * (own[C] - org[C]);
-    const R coef = R(P.coef[AX]);
-    Flux<N, R> F;
-    if (own[AX] > 0) {
-      flux_from_lds<MODEL, N, D, R>(s_flux + (l * (T + 1) + t) * SZ::NF, F);
-      add_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
-    }
-    if (own[AX] < na - 1) {
-      flux_from_lds<MODEL, N, D, R>(s_flux + (l * (T + 1) + t + 1) * SZ::NF, F);
-      sub_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
-    }
-  }
-  __syncthreads();
 }
 
 template <int MODEL, int N, int D, int FR, typename R>
@@ -321,7 +345,7 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 
 // Wave-aggregated append of relaxation samples (one atomic per wave).
 __device__ __forceinline__ void capture_samples(const SampleSink& K, bool cap, const float* feat, float target,
-                                                int iters, float rel_spread) {
+                                                int iters, float remainder, float sigma) {
   const unsigned long long mask = __ballot(cap);
   if (mask == 0ull) return;
   const int lane = threadIdx.x & 63;
@@ -340,19 +364,17 @@ __device__ __forceinline__ void capture_samples(const SampleSink& K, bool cap, c
   o[0] = make_float4(feat[0], feat[1], feat[2], feat[3]);
   o[1] = make_float4(feat[4], feat[5], feat[6], feat[7]);
   o[2] = make_float4(feat[8], feat[9], feat[10], feat[11]);
-  o[3] = make_float4(target, (float)iters, rel_spread, 0.f);
+  o[3] = make_float4(target, (float)iters, remainder, sigma);
 }
 
-// LDS bytes one wave uses to stage the network's inputs and outputs.
-constexpr int MLP_STAGE_BYTES = 64 * 16 * 2 + 64 * 4;
-
 template <int MODEL, int N, int D, bool O2, typename R, bool LEARN>
-__global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock* __restrict__ clk,
+__global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clock* __restrict__ clk,
                                               Counters* __restrict__ cnt) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   R* lds = reinterpret_cast<R*>(lds_raw);
   __shared__ double s_dt[4], s_res[4];
   __shared__ unsigned long long s_c[5][4];
+  __shared__ int s_mlp[4];
   if (clk->done) return;
   const double dt = clk->dt;
   int org[3];
@@ -368,15 +390,13 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock*
   R dtm = R(1.e10);
   unsigned long long n_it = 0, n_fail = 0, n_relax = 0, n_warm = 0, n_wfb = 0;
   double resid = 0.;
-  // Time-n cell: non-conservative terms and U^n (Cell::getPhase default type).
-  Cell<N, R> ownR;
+  // Time-n phases of the own cell for the non-conservative terms (Cell::getPhase default type).
   R an[N], pn[N];
 #pragma unroll
   for (int q = 0; q < N; q++) { an[q] = R(0.); pn[q] = R(0.); }
-  if (own_ok) {
-    load_prim<MODEL, N, D>(P, L.wn, c, ownR);
+  if (own_ok && MODEL != M_EULER) {
 #pragma unroll
-    for (int q = 0; q < N; q++) { an[q] = ownR.a[q]; pn[q] = ownR.p[q]; }
+    for (int q = 0; q < N; q++) { an[q] = R(L.wn[(3 * q) * P.pitch + c]); pn[q] = R(L.wn[(3 * q + 2) * P.pitch + c]); }
   }
   ConsAcc<N, R> acc;
 
@@ -386,7 +406,7 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock*
 
   Cell<N> nw;
   bool relax_now = false, spread_ok = false;
-  double p_cold = 0., spread = 0.;
+  double p_cold = 0., spread = 0., sigma = 1e-9, t_lin_d = 0.;
   float feat[MLP_IN];
   float t_lin = 0.f;
 #pragma unroll
@@ -394,8 +414,7 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock*
   if (own_ok) {
     // Boundary faces see the stage's own cell: W^n (first order / predictor) or W* (corrector).
     Cell<N, R> src;
-    if (O2 && L.stage == 2) load_prim<MODEL, N, D>(P, L.wsrc, c, src);
-    else src = ownR;
+    load_prim<MODEL, N, D>(P, (O2 && L.stage == 2) ? L.wsrc : L.wn, c, src);
     derive<MODEL, N, R>(P, src, false);
     const int i = own[0], j = own[1], k = own[2];
     if (i == 0) boundary_face<MODEL, N, D, FR_XM, R>(P, src, P.bc[0], R(P.coef[0]), an, pn, dtm, acc);
@@ -433,17 +452,15 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock*
     if (MODEL != M_EULER && P.relax) {
       relax_now = !relax_gated<N>(P, nw);
       if (relax_now && (LEARN || L.sink.buf != nullptr))
-        spread_ok = relax_features<N>(P, nw, p_cold, spread, feat, t_lin);
+        spread_ok = relax_features<N>(P, nw, p_cold, spread, feat, t_lin, t_lin_d, sigma);
     }
   }
 
-  // Learned warm start: the whole wave runs the network when any of its cells
-  // relaxes from distinct phase pressures.
+  // Learned warm start for the block's cells that relax from distinct phase pressures.
   float y = 0.f;
   if constexpr (LEARN && MODEL != M_EULER) {
     __syncthreads();   // the tile's LDS is free from here on
-    if (__ballot(spread_ok) != 0ull)
-      y = mlp_infer_wave(L.mlp, feat, reinterpret_cast<unsigned short*>(lds_raw + (tid >> 6) * MLP_STAGE_BYTES));
+    y = mlp_block_infer(L.mlp, spread_ok, feat, lds_raw, s_mlp);
   }
 
   int it = 0;
@@ -453,7 +470,7 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock*
     double fres = 0.;
     if (LEARN && spread_ok) {
       bool fell = false;
-      const double guess = p_cold + spread * ((double)t_lin + (double)y);
+      const double guess = p_cold + spread * (t_lin_d + sigma * (double)y);
       it = relax_warm<N>(P, nw, guess, p_cold, WARM_MAX_IT, fell, fres);
       n_warm = 1;
       n_wfb = fell ? 1 : 0;
@@ -466,9 +483,9 @@ __global__ __launch_bounds__(256) void k_step(Params P, StepLaunchT<R> L, Clock*
   }
   if (MODEL != M_EULER && L.sink.buf != nullptr) {
     const bool cap = spread_ok && it > 0 && it < 100;
-    const float target = cap ? (float)((nw.p[0] - p_cold) / spread) : 0.f;
-    const float rel = cap ? (float)(spread / fmax(fabs(nw.p[0]), 1.e-300)) : 0.f;
-    capture_samples(L.sink, cap, feat, target, it, rel);
+    const double t = cap ? (nw.p[0] - p_cold) / spread : 0.;
+    const float rem = cap ? (float)((t - t_lin_d) / sigma) : 0.f;
+    capture_samples(L.sink, cap, feat, (float)t, it, rem, (float)sigma);
   }
   if (own_ok) {
     if (MODEL != M_EULER && P.relax) correct_and_derive<MODEL, N>(P, nw);
@@ -576,8 +593,8 @@ template <int MODEL, int N, int D, bool O2, typename R, bool LEARN>
 void launch_step_v(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counters* cnt, hipStream_t s) {
   using SZ = Sizes<MODEL, N, D>;
   const int blocks = P.tiles[0] * P.tiles[1] * P.tiles[2];
-  size_t lds = sizeof(R) * SZ::lds_elems(O2);
-  if (LEARN && lds < (size_t)(4 * MLP_STAGE_BYTES)) lds = 4 * MLP_STAGE_BYTES;
+  size_t lds = sizeof(R) * SZ::lds_elems(O2, (int)sizeof(R));
+  if (LEARN && lds < (size_t)MLP_BLOCK_LDS) lds = MLP_BLOCK_LDS;
   auto k = k_step<MODEL, N, D, O2, R, LEARN>;
   hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);

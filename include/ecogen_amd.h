@@ -175,9 +175,11 @@ int ecg_state_diff(ecg_ctx* a, ecg_ctx* b, double* out, int nfields);
  *
  * A sample record is ECG_SAMPLE_FLOATS floats: 12 scale-free features of the
  * pre-relaxation phase states (3 phase slots x [alpha_k, (p_k - p_cold)/s,
- * linearised weight, s/(s + rho_k c_k^2)], s = max_k |p_k - p_cold|), the
+ * linearised weight w_k, s/(s + rho_k c_k^2)], s = max_k |p_k - p_cold|), the
  * normalised relaxed pressure t = (p* - p_cold)/s, the cold-start Newton
- * evaluations, s/|p*|, and one spare float. */
+ * evaluations, the remainder u = (t - t_lin)/sigma computed in FP64, and the
+ * second-order scale sigma = max(|sum_k w_k (t_lin - x_k1)^2 s/(rho_k c_k^2)|,
+ * 1e-9), t_lin = sum_k w_k x_k1. */
 #define ECG_MLP_INPUTS 12
 #define ECG_MLP_HIDDEN 64
 #define ECG_MLP_NPARAM 5057    /* W1[12][64], b1[64], W2[64][64], b2[64], w3[64], b3 */
@@ -202,7 +204,7 @@ int ecg_samples_upload(ecg_samples* s, const float* host, int64_t n);
  * serves one attached context at a time.  NULL detaches. */
 int ecg_attach_samples(ecg_ctx* ctx, ecg_samples* s, int stride);
 
-/* 12 -> 64 -> 64 -> 1 GELU network predicting t - t_lin, t_lin = sum_k x_k2 x_k1.
+/* 12 -> 64 -> 64 -> 1 GELU network predicting u = (t - t_lin) / sigma (clipped to +-4 in training).
  * Parameters are FP32 on the device; inference packs them as BF16 for MFMA. */
 ecg_mlp* ecg_mlp_create(uint64_t seed);
 void ecg_mlp_destroy(ecg_mlp* m);
@@ -214,9 +216,10 @@ int ecg_mlp_gradient(ecg_mlp* m, ecg_samples* s, int64_t n, float* grad, double*
  * receives the mean loss of epoch e. */
 int ecg_mlp_train(ecg_mlp* m, ecg_samples* s, int epochs, int batch, float lr, uint64_t seed,
                   double* epoch_loss);
-/* BF16-MFMA inference of t (= t_lin + network) for the first n records. */
+/* BF16-MFMA inference (the device function fused into the relaxation): the network
+ * output y for the first n records (the warm start uses t = t_lin + sigma * y). */
 int ecg_mlp_predict(ecg_mlp* m, ecg_samples* s, int64_t n, float* out);
-/* Use m's guess, p = p_cold + s (t_lin + y), in the relaxations of an
+/* Use m's guess, p = p_cold + s (t_lin + sigma y), in the relaxations of an
  * ECG_FP32_LEARNED context; Newton still runs to |f| <= 1e-10 and restarts from
  * the cold guess after 30 evaluations.  NULL returns to the cold start. */
 int ecg_set_warm_start(ecg_ctx* ctx, const ecg_mlp* m);

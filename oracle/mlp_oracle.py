@@ -67,12 +67,22 @@ def forward(prm, x):
     return h2 @ w3 + b3
 
 
+CLIP = 4.0
+
+
+def target(rec):
+    """Training target of each record: the stored remainder u = (t - t_lin) / sigma
+    clipped to [-4, 4]."""
+    rec = np.asarray(rec, dtype=np.float64)
+    return np.clip(rec[:, 14], -CLIP, CLIP)
+
+
 def loss_and_grad(prm, rec):
-    """Mean of (y - (t - t_lin))^2 over the records and its gradient, float64."""
+    """Mean of (y - target)^2 over the records and its gradient, float64."""
     W1, b1, W2, b2, w3, b3 = unpack(prm)
     rec = np.asarray(rec, dtype=np.float64)
     x = rec[:, :N_IN]
-    r = rec[:, 12] - t_lin(x)
+    r = target(rec)
     n = x.shape[0]
     z1 = x @ W1 + b1
     h1 = gelu(z1)
@@ -104,24 +114,25 @@ def bf16(a):
 
 def forward_bf16(prm, x):
     """Inference as the MFMA path computes it: BF16 inputs, weights and hidden
-    activations, wide accumulation, FP32 biases; returns t_lin + y."""
+    activations, wide accumulation, FP32 biases; returns the network output y."""
     W1, b1, W2, b2, w3, b3 = unpack(np.asarray(prm, dtype=np.float32))
     x = np.asarray(x, dtype=np.float32)
     xb = bf16(x)
     h1 = bf16(gelu(xb @ bf16(W1) + b1))
     h2 = bf16(gelu(h1 @ bf16(W2) + b2))
-    y = h2 @ bf16(w3) + b3
-    return t_lin(x.astype(np.float64)) + y
+    return h2 @ bf16(w3) + b3
 
 
 def features(alpha, p, eos):
     """Features, cold guess, spread and t_lin of one cell's phase states.
 
     alpha, p: per phase (N = len(eos)); eos: dicts with gamma and pinf (0 for IG).
-    Returns (x[12], p_cold, s, t_lin, f_lin) or None when all phases share one
-    pressure; f_lin = sum_k alpha_k (p_k - p_cold) / Z_k is the linearised
-    residual at the cold guess (the library consults the network, and records
-    samples, only where |f_lin| > 1e-11)."""
+    Returns (x[12], p_cold, s, t_lin, f_lin, sigma) or None when all phases
+    share one pressure; f_lin = sum_k alpha_k (p_k - p_cold) / Z_k is the
+    linearised residual at the cold guess (the library consults the network,
+    and records samples, only where |f_lin| > 1e-11); sigma =
+    max(|sum_k w_k (t_lin - x_k1)^2 s / Z_k|, 1e-9) is the second-order scale
+    of t - t_lin."""
     n = len(eos)
     pc = 0.
     for k in range(n):
@@ -141,4 +152,8 @@ def features(alpha, p, eos):
         wk = w[k] / ws
         x[4 * k:4 * k + 4] = (alpha[k], dp, wk, s / (s + Z[k]))
         tl += wk * dp
-    return x, pc, s, tl, fl
+    r2 = 0.
+    for k in range(n):
+        d = tl - (p[k] - pc) / s
+        r2 += (w[k] / ws) * d * d * (s / Z[k])
+    return x, pc, s, tl, fl, max(abs(r2), 1e-9)

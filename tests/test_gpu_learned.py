@@ -29,7 +29,10 @@ def _random_records(rng, n):
         p = 10 ** rng.uniform(4.5, 9, size=k)
         f = mo.features(a, p, EOS3[:k])
         rec[i, :12] = f[0]
-        rec[i, 12] = f[3] + rng.normal(0, 0.05)
+        u = rng.normal(0, 1.5)
+        rec[i, 15] = f[5]
+        rec[i, 14] = u
+        rec[i, 12] = f[3] + f[5] * u
     return rec
 
 
@@ -51,7 +54,7 @@ def test_capture_matches_oracle_relaxation(model, eos, alpha, rho, p):
     s.step(1.e-15)
     rec = smp.download()
     assert rec.shape[0] == case["nx"]
-    x, pc, sp, tl, fl = mo.features(alpha, p, eos)
+    x, pc, sp, tl, fl, sg = mo.features(alpha, p, eos)
     assert abs(fl) > 1e-11
     L = po.lib()
     n = len(eos)
@@ -64,6 +67,10 @@ def test_capture_matches_oracle_relaxation(model, eos, alpha, rho, p):
                                rtol=0, atol=2e-7)
     np.testing.assert_allclose(rec[:, 12], t, rtol=2e-6, atol=2e-7)
     assert np.all(rec[:, 13] == it)
+    np.testing.assert_allclose(rec[:, 15], sg, rtol=1e-6)
+    # remainder over the linearised root in units of sigma; the root is known to the Newton tolerance
+    Zmax = max(e["gamma"] * (p[k] + e.get("pinf", 0.)) for k, e in enumerate(eos))
+    np.testing.assert_allclose(rec[:, 14], (t - tl) / sg, rtol=1e-5, atol=2e-10 * Zmax / sp / sg)
 
 
 @pytest.mark.gpu
@@ -99,7 +106,7 @@ def test_mfma_inference_matches_bf16_oracle():
     m.set_params(prm)
     out = m.predict(smp)
     ref = mo.forward_bf16(prm, rec[:, :12])
-    ref64 = mo.t_lin(rec[:, :12]) + mo.forward(prm.astype(np.float64), rec[:, :12].astype(np.float64))
+    ref64 = mo.forward(prm.astype(np.float64), rec[:, :12].astype(np.float64))
     d = np.abs(out - ref)
     scale = 1.0 + np.abs(ref)
     print(f"[learned] inference |gpu - bf16 oracle| max {np.max(d / scale):.2e} mean {np.mean(d / scale):.2e}; "

@@ -19,7 +19,10 @@ def _random_records(rng, n):
         p = 10 ** rng.uniform(4.5, 9, size=k)
         f = mo.features(a, p, EOS3[:k])
         rec[i, :12] = f[0]
-        rec[i, 12] = f[3] + rng.normal(0, 0.05)
+        u = rng.normal(0, 1.5)
+        rec[i, 15] = f[5]
+        rec[i, 14] = u
+        rec[i, 12] = f[3] + f[5] * u
     return rec
 
 
@@ -46,7 +49,7 @@ def test_forward_agrees_with_loss_path():
     prm = rng.normal(0, 0.2, mo.N_PARAM)
     rec = _random_records(rng, 25)
     y = mo.forward(prm, rec[:, :12])
-    r = rec[:, 12] - mo.t_lin(rec[:, :12])
+    r = mo.target(rec)
     assert abs(np.mean((y - r) ** 2) - mo.loss_and_grad(prm, rec)[0]) <= 1e-14 * (1 + np.mean(r * r))
 
 
@@ -57,7 +60,7 @@ def test_tlin_is_the_linearised_root():
         k = rng.integers(2, 4)
         a = rng.dirichlet(np.ones(k))
         p = 10 ** rng.uniform(4, 9.5, size=k)
-        x, pc, s, tl, _ = mo.features(a, p, EOS3[:k])
+        x, pc, s, tl, _, _ = mo.features(a, p, EOS3[:k])
         Z = [e["gamma"] * (p[j] + e["pinf"]) for j, e in enumerate(EOS3[:k])]
         pl = pc + s * tl
         res = sum(a[j] * (pl - p[j]) / Z[j] for j in range(k))
@@ -81,7 +84,7 @@ def test_relaxed_root_tends_to_tlin_for_small_spreads():
         base = 10 ** rng.uniform(5, 8)
         eps_rel = 10 ** rng.uniform(-6, -3)
         p = base * (1 + eps_rel * rng.uniform(-1, 1, size=k))
-        x, pc, s, tl, _ = mo.features(a, p, EOS3)
+        x, pc, s, tl, _, sg = mo.features(a, p, EOS3)
         aa, rr, pp, ps = a.copy(), rho0.copy(), p.copy(), np.zeros(1)
         it = L.orc_relax_cell(k, eos, 0.0, 0, aa.ctypes.data_as(DP), rr.ctypes.data_as(DP),
                               pp.ctypes.data_as(DP), ps.ctypes.data_as(DP))
@@ -91,3 +94,33 @@ def test_relaxed_root_tends_to_tlin_for_small_spreads():
         # Newton tolerance in pressure is ~1e-10 * Z; the nonlinearity is O(s / Z)
         assert abs(t - tl) <= 20 * s / Zmin + 2e-10 * max(e["gamma"] * (p[j] + e["pinf"])
                                                            for j, e in enumerate(EOS3)) / s
+
+
+def test_sigma_is_the_second_order_remainder():
+    """For small spreads t - t_lin approaches the second-order term of the
+    rational relaxation equation, sum_k w_k (t_lin - x_k1)^2 s / (p_k + pinf_k),
+    which is sigma (computed with Z_k = gamma_k (p_k + pinf_k)) times gamma-weighted
+    factors in [min gamma, max gamma]."""
+    L = po.lib()
+    eos = (po.OrcEos * 3)(*[po.make_eos(e) for e in EOS3])
+    rng = np.random.default_rng(13)
+    rho0 = np.array([1000., 1185., 1.2])
+    g = np.array([e["gamma"] for e in EOS3])
+    for _ in range(100):
+        a = rng.dirichlet(np.ones(3))
+        a = np.clip(a, 1e-2, None)
+        a /= a.sum()
+        base = 10 ** rng.uniform(6, 8)
+        p = base * (1 + 1e-3 * rng.uniform(-1, 1, size=3))
+        x, pc, s, tl, _, sg = mo.features(a, p, EOS3)
+        aa, rr, pp, ps = a.copy(), rho0.copy(), p.copy(), np.zeros(1)
+        it = L.orc_relax_cell(3, eos, 0.0, 0, aa.ctypes.data_as(DP), rr.ctypes.data_as(DP),
+                              pp.ctypes.data_as(DP), ps.ctypes.data_as(DP))
+        assert 0 < it < 100
+        rem = (ps[0] - pc) / s - tl
+        # second-order term with the exact s/(p_k + pinf_k) = gamma_k s / Z_k
+        r2 = sum(x[4 * k + 2] * (tl - x[4 * k + 1]) ** 2 * g[k] * (s / (EOS3[k]["gamma"] * (p[k] + EOS3[k]["pinf"])))
+                 for k in range(3))
+        tol = 1e-10 * max(e["gamma"] * (p[k] + e["pinf"]) for k, e in enumerate(EOS3)) / s
+        assert abs(rem - r2) <= 0.05 * abs(r2) + tol
+        assert g.min() * sg * 0.9 - tol <= abs(r2) <= g.max() * sg * 1.1 + tol
