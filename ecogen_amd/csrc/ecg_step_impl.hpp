@@ -131,15 +131,18 @@ __device__ __forceinline__ void slope_vars(const Cell<N, R>& L, const Cell<N, R>
 // uniform Cartesian cell: slope1 is the face's own slope, slope2 the opposite
 // face's slope when that face is interior (boundary faces carry none: slope2
 // and the opposite alpha stay 0).  negate: right cell (changeSign,
-// CellInterfaceO2Cartesian.cpp:151-156).
-template <int MODEL, int N, int D, typename R>
+// CellInterfaceO2Cartesian.cpp:151-156).  LK >= 0: every limiter of the case is
+// of kind LK (no interface switch to evaluate); LK < 0: kinds from P.lim.
+template <int MODEL, int N, int D, typename R, int LK>
 __device__ __forceinline__ void extrapolate(const Params& P, Cell<N, R>& b, const R* s1, const R* s2, bool has2,
                                             R aAfter, R aCell, R aOther, R dist, bool negate) {
-  int lg, lvf;
-  if ((aCell >= R(1.e-4)) && (aCell <= R(1.) - R(1.e-4)) && ((aOther - aCell) * (aCell - aAfter) >= R(1.e-8))) {
-    lg = P.lim[1]; lvf = P.lim[3];
-  } else {
-    lg = P.lim[0]; lvf = P.lim[2];
+  int lg = LK, lvf = LK;
+  if (LK < 0) {
+    if ((aCell >= R(1.e-4)) && (aCell <= R(1.) - R(1.e-4)) && ((aOther - aCell) * (aCell - aAfter) >= R(1.e-8))) {
+      lg = P.lim[1]; lvf = P.lim[3];
+    } else {
+      lg = P.lim[0]; lvf = P.lim[2];
+    }
   }
   int v = 0;
 #pragma unroll
@@ -222,7 +225,7 @@ __device__ __forceinline__ void tile_origin(const Params& P, int& i0, int& j0, i
 
 // One axis pass (steps A, B, C above) for axis AX with face frame FR, in
 // batches of lines when the staging would exceed LDS_BATCH_BYTES.
-template <int MODEL, int N, int D, bool O2, typename R, int AX, int FR>
+template <int MODEL, int N, int D, bool O2, typename R, int LK, int AX, int FR>
 __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>& L, R* lds, const int org[3],
                                           const int own[3], bool own_ok, const R* an, const R* pn, R& dtm,
                                           ConsAcc<N, R>& acc) {
@@ -282,10 +285,11 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
           const R aR = (MODEL == M_EULER) ? R(0.) : br.a[0];
           const R* s1 = s_slope + (lb * (T + 3) + f + 2) * SZ::NS;
           const bool hasm = (m >= 1), hasp = (m + 1 <= na - 2);
-          const R aLL = (MODEL != M_EULER && hasm) ? alpha0_at<R>(L.wn, cell - sa) : R(0.);
-          const R aRR = (MODEL != M_EULER && hasp) ? alpha0_at<R>(L.wn, cell + 2 * sa) : R(0.);
-          extrapolate<MODEL, N, D, R>(P, bl, s1, s1 - SZ::NS, hasm, aLL, aL, aR, R(P.dfl[AX][m]), false);
-          extrapolate<MODEL, N, D, R>(P, br, s1, s1 + SZ::NS, hasp, aRR, aR, aL, R(P.dfr[AX][m]), true);
+          // The far neighbours' time-n alpha only decides the interface limiter switch.
+          const R aLL = (LK < 0 && MODEL != M_EULER && hasm) ? alpha0_at<R>(L.wn, cell - sa) : R(0.);
+          const R aRR = (LK < 0 && MODEL != M_EULER && hasp) ? alpha0_at<R>(L.wn, cell + 2 * sa) : R(0.);
+          extrapolate<MODEL, N, D, R, LK>(P, bl, s1, s1 - SZ::NS, hasm, aLL, aL, aR, R(P.dfl[AX][m]), false);
+          extrapolate<MODEL, N, D, R, LK>(P, br, s1, s1 + SZ::NS, hasp, aRR, aR, aL, R(P.dfr[AX][m]), true);
           derive<MODEL, N, R>(P, bl, MODEL == M_EULER);
           derive<MODEL, N, R>(P, br, MODEL == M_EULER);
         } else {
@@ -367,7 +371,7 @@ __device__ __forceinline__ void capture_samples(const SampleSink& K, bool cap, c
   o[3] = make_float4(target, (float)iters, remainder, sigma);
 }
 
-template <int MODEL, int N, int D, bool O2, typename R, bool LEARN>
+template <int MODEL, int N, int D, bool O2, typename R, bool LEARN, int LK>
 __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clock* __restrict__ clk,
                                               Counters* __restrict__ cnt) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -400,9 +404,9 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
   }
   ConsAcc<N, R> acc;
 
-  axis_pass<MODEL, N, D, O2, R, 0, FR_XI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
-  if (D >= 2) axis_pass<MODEL, N, D, O2, R, 1, FR_YI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
-  if (D >= 3) axis_pass<MODEL, N, D, O2, R, 2, FR_ZI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
+  axis_pass<MODEL, N, D, O2, R, LK, 0, FR_XI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
+  if (D >= 2) axis_pass<MODEL, N, D, O2, R, LK, 1, FR_YI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
+  if (D >= 3) axis_pass<MODEL, N, D, O2, R, LK, 2, FR_ZI>(P, L, lds, org, own, own_ok, an, pn, dtm, acc);
 
   Cell<N> nw;
   bool relax_now = false, spread_ok = false;
@@ -589,15 +593,25 @@ __global__ __launch_bounds__(256) void k_fill(Params P, const DomainDev* __restr
   store_prim<MODEL, N, D>(P, W, c, s);
 }
 
-template <int MODEL, int N, int D, bool O2, typename R, bool LEARN>
+template <int MODEL, int N, int D, bool O2, typename R, bool LEARN, int LK>
 void launch_step_v(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counters* cnt, hipStream_t s) {
   using SZ = Sizes<MODEL, N, D>;
   const int blocks = P.tiles[0] * P.tiles[1] * P.tiles[2];
   size_t lds = sizeof(R) * SZ::lds_elems(O2, (int)sizeof(R));
   if (LEARN && lds < (size_t)MLP_BLOCK_LDS) lds = MLP_BLOCK_LDS;
-  auto k = k_step<MODEL, N, D, O2, R, LEARN>;
+  auto k = k_step<MODEL, N, D, O2, R, LEARN, LK>;
   hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
+}
+
+// Second order: a case whose four limiters are all minmod (the common choice)
+// runs the instantiation with the limiter fixed at compile time.
+template <int MODEL, int N, int D, typename R, bool LEARN>
+void launch_step_o2(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counters* cnt, hipStream_t s) {
+  const bool all_minmod = P.lim[0] == LIM_MINMOD && P.lim[1] == LIM_MINMOD && P.lim[2] == LIM_MINMOD &&
+                          P.lim[3] == LIM_MINMOD;
+  if (all_minmod) launch_step_v<MODEL, N, D, true, R, LEARN, LIM_MINMOD>(P, L, clk, cnt, s);
+  else launch_step_v<MODEL, N, D, true, R, LEARN, -1>(P, L, clk, cnt, s);
 }
 
 template <int MODEL, int N, int D, typename R>
@@ -611,14 +625,14 @@ void launch_step_t(const Params& P, const StepLaunch& Lv, Clock* clk, Counters* 
   const bool learn = CAN_LEARN && Lv.mlp.frag != nullptr;
   if (Lv.stage == 0) {
     if constexpr (CAN_LEARN) {
-      if (learn) { launch_step_v<MODEL, N, D, false, R, true>(P, L, clk, cnt, s); return; }
+      if (learn) { launch_step_v<MODEL, N, D, false, R, true, -1>(P, L, clk, cnt, s); return; }
     }
-    launch_step_v<MODEL, N, D, false, R, false>(P, L, clk, cnt, s);
+    launch_step_v<MODEL, N, D, false, R, false, -1>(P, L, clk, cnt, s);
   } else {
     if constexpr (CAN_LEARN) {
-      if (learn) { launch_step_v<MODEL, N, D, true, R, true>(P, L, clk, cnt, s); return; }
+      if (learn) { launch_step_o2<MODEL, N, D, R, true>(P, L, clk, cnt, s); return; }
     }
-    launch_step_v<MODEL, N, D, true, R, false>(P, L, clk, cnt, s);
+    launch_step_o2<MODEL, N, D, R, false>(P, L, clk, cnt, s);
   }
 }
 

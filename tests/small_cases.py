@@ -121,10 +121,20 @@ def pueq_2d_xcd(order=1):
     return c
 
 
+def mixed_limiters(case, lims=("minmod", "superbee", "vanleer", "mc")):
+    """Distinct global / interface / volume-fraction limiters: the interface switch
+    (CellO2Cartesian.cpp:150-170) decides which one each face side uses."""
+    c = dict(case)
+    c["limiters"] = lims
+    c["name"] = case["name"] + "_mixlim"
+    return c
+
+
 def all_cases():
     cs = [pueq_3d_tiles(2), pueq_2d_xcd(1), pueq_2d_xcd(2), euler_sod(1), euler_sod(2), euler_2d(1), euler_2d(2), euler_2d(2, lim="superbee"), euler_3d(1),
           euler_3d(2), wall_variant(euler_2d(1)), wall_variant(euler_3d(1)),
           pueq_1d(1), pueq_1d(2), pueq_2d(1), pueq_2d(2), pueq_2d(1, alpha_null=True), wall_variant(pueq_2d(1)),
           pueq3_3d(1), pueq3_3d(2),
-          ueq_2d(1), ueq_2d(2), ueq_2d(1, alpha_null=True), ueq3_3d(1), ueq2_3d(2), wall_variant(ueq3_3d(1))]
+          ueq_2d(1), ueq_2d(2), ueq_2d(1, alpha_null=True), ueq3_3d(1), ueq2_3d(2), wall_variant(ueq3_3d(1)),
+          mixed_limiters(pueq_2d(2)), mixed_limiters(ueq2_3d(2), ("vanalbada", "minmod", "mc", "superbee"))]
     return {c["name"]: c for c in cs}
