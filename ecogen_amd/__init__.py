@@ -47,6 +47,11 @@ class EcgDomain(C.Structure):
                 ("p", C.c_double * MAX_PHASES), ("vel", C.c_double * 3)]
 
 
+class EcgRunControl(C.Structure):
+    _fields_ = [("run_name", C.c_char * 128), ("iterations_mode", C.c_int), ("iterations", C.c_int64),
+                ("iter_freq", C.c_int), ("total_time", C.c_double), ("time_freq", C.c_double)]
+
+
 class EcgStepStats(C.Structure):
     _fields_ = [("t", C.c_double), ("dt_next", C.c_double), ("dt_max", C.c_double), ("iteration", C.c_int64),
                 ("newton_fail", C.c_int64), ("newton_iters", C.c_int64), ("relax_calls", C.c_int64),
@@ -80,6 +85,8 @@ SIGNATURES = {
     "ecg_kernel_times": (C.c_int, [C.c_void_p, _DP, C.POINTER(C.c_int64), C.c_int]),
     "ecg_calibration_copy": (C.c_int, [C.c_void_p, C.c_int64]),
     "ecg_state_diff": (C.c_int, [C.c_void_p, C.c_void_p, _DP, C.c_int]),
+    "ecg_read_case": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(EcgConfig), C.POINTER(EcgDomain), C.c_int,
+                                C.POINTER(C.c_int), C.POINTER(EcgRunControl)]),
     "ecg_samples_create": (C.c_void_p, [C.c_int64]),
     "ecg_samples_destroy": (None, [C.c_void_p]),
     "ecg_samples_reset": (C.c_int, [C.c_void_p]),
@@ -172,6 +179,43 @@ def make_domains(case):
             o.rho[k] = d["rho"][k]
             o.p[k] = d["p"][k]
     return arr
+
+
+def read_case(case_dir, eos_dir=None, precision="fp64"):
+    """A case directory in the reference's XML input format (main.xml, mesh.xml,
+    model.xml, initialConditions.xml + libEOS files), read by the library's
+    ecg_read_case, as a case dict; run control under "control"."""
+    L = library()
+    cfg = EcgConfig()
+    rc = EcgRunControl()
+    nd = C.c_int(0)
+    args = (str(case_dir).encode(), eos_dir.encode() if eos_dir else None, PRECISIONS[precision])
+    r = L.ecg_read_case(*args, C.byref(cfg), None, 0, C.byref(nd), C.byref(rc))
+    if r < 0 and nd.value == 0:
+        _check(r)
+    doms = (EcgDomain * max(nd.value, 1))()
+    _check(L.ecg_read_case(*args, C.byref(cfg), doms, nd.value, C.byref(nd), C.byref(rc)))
+    inv = lambda m, v: next(k for k, x in m.items() if x == v)
+    eos = []
+    for k in range(cfg.nph):
+        e = cfg.eos[k]
+        eos.append(dict(type=inv(EOS_TYPES, e.type), gamma=e.gamma, pinf=e.pinf, cv=e.cv, eref=e.eref))
+    kinds = {v: k for k, v in DOMAINS.items()}
+    domains = []
+    for i in range(nd.value):
+        d = doms[i]
+        domains.append(dict(kind=kinds[d.kind], axis=d.axis, axis2=d.axis2, dir=d.dir, origin=d.origin,
+                            radius=d.radius, center=tuple(d.center), lo=tuple(d.lo), len=tuple(d.len),
+                            vel=tuple(d.vel), alpha=list(d.alpha[:cfg.nph]), rho=list(d.rho[:cfg.nph]),
+                            p=list(d.p[:cfg.nph])))
+    bcs = ("nonreflecting", "wall")
+    return dict(name=rc.run_name.decode(), nx=cfg.nx, ny=cfg.ny, nz=cfg.nz, lx=cfg.lx, ly=cfg.ly, lz=cfg.lz,
+                model=inv(MODELS, cfg.model), eos=eos, order=cfg.order,
+                limiters=tuple(inv(LIMITERS, cfg.lim[i]) for i in range(4)), alpha_null=bool(cfg.alpha_null),
+                relax_p=bool(cfg.relax_p), bc=tuple(bcs[cfg.bc[i]] for i in range(6)), cfl=cfg.cfl,
+                precision=precision, domains=domains,
+                control=dict(iterations_mode=bool(rc.iterations_mode), iterations=rc.iterations,
+                             iter_freq=rc.iter_freq, total_time=rc.total_time, time_freq=rc.time_freq))
 
 
 class Solver:

@@ -28,9 +28,15 @@ COMMON = [
 ]
 
 
+HOST = os.path.join(HERE, "host")
+HOST_SOURCES = ["xml.cpp", "ecogen.cpp"]
+RUNNER = os.path.join(HERE, "ecogen_run")
+
+
 def _compile(src):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
+    obj = os.path.join(BUILD, os.path.splitext(os.path.basename(src))[0] + ".o")
+    path = src if os.path.isabs(src) else os.path.join(CSRC, src)
+    cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
@@ -40,16 +46,24 @@ def _compile(src):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     jobs = min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)))
+    srcs = SOURCES + [os.path.join(HOST, h) for h in HOST_SOURCES]
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # Command-line runner for cases in the reference's XML format.
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", os.path.join(HOST, "ecogen_run.cpp"), "-o",
+           RUNNER + ".tmp", "-L" + HERE, "-lecogen_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"runner build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(RUNNER + ".tmp", RUNNER)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {RUNNER}")
     return LIB
 
 

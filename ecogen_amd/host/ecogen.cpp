@@ -1,0 +1,486 @@
+// Case input (reference XML format) and the host Run object; see ecogen.hpp.
+#include "ecogen.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <sys/stat.h>
+
+#include "xml.hpp"
+
+namespace ecg {
+int api_fail(int code, const std::string& msg);
+}
+
+namespace ecg_host {
+
+namespace {
+
+std::string upper(std::string s) {
+  for (auto& c : s) c = (char)std::toupper((unsigned char)c);
+  return s;
+}
+
+bool is_dir(const std::string& p) {
+  struct stat st;
+  return stat(p.c_str(), &st) == 0 && S_ISDIR(st.st_mode);
+}
+
+std::string join(const std::string& a, const std::string& b) {
+  if (a.empty()) return b;
+  return (a.back() == '/') ? a + b : a + "/" + b;
+}
+
+[[noreturn]] void bad(const std::string& file, const std::string& msg) {
+  throw InputError(ECG_E_ARG, file + ": " + msg);
+}
+[[noreturn]] void unsupported(const std::string& file, const std::string& msg) {
+  throw InputError(ECG_E_UNSUPPORTED, file + ": " + msg);
+}
+
+const XmlElement& need(const XmlElement* e, const std::string& name, const std::string& file) {
+  if (!e) bad(file, "missing element <" + name + ">");
+  return *e;
+}
+
+bool has(const XmlElement& e, const char* a) { return e.attr(a) != nullptr; }
+
+double num(const XmlElement& e, const char* a, const std::string& file) {
+  const std::string* v = e.attr(a);
+  if (!v) bad(file, "<" + e.name + "> needs attribute " + a);
+  const char* s = v->c_str();
+  char* end = nullptr;
+  errno = 0;
+  const double d = std::strtod(s, &end);
+  while (end && *end && std::isspace((unsigned char)*end)) end++;
+  if (end == s || (end && *end) || errno == ERANGE) bad(file, "<" + e.name + "> attribute " + a + " is not a number: " + *v);
+  return d;
+}
+
+long long integer(const XmlElement& e, const char* a, const std::string& file) {
+  const double d = num(e, a, file);
+  if (d != std::floor(d)) bad(file, "<" + e.name + "> attribute " + a + " must be an integer");
+  return (long long)d;
+}
+
+bool boolean(const XmlElement& e, const char* a, const std::string& file) {
+  const std::string* v = e.attr(a);
+  if (!v) bad(file, "<" + e.name + "> needs attribute " + a);
+  const std::string u = upper(*v);
+  if (u == "TRUE" || u == "1") return true;
+  if (u == "FALSE" || u == "0") return false;
+  bad(file, "<" + e.name + "> attribute " + a + " must be true or false");
+}
+
+int axis_of(const std::string& s, const std::string& file) {
+  const std::string u = upper(s);
+  if (u == "X") return 0;
+  if (u == "Y") return 1;
+  if (u == "Z") return 2;
+  bad(file, "unknown axis " + s);
+}
+
+std::string find_eos_dir(const std::string& caseDir) {
+  std::string d = caseDir;
+  for (int up = 0; up < 12; up++) {
+    if (is_dir(join(d, "libEOS"))) return join(d, "libEOS");
+    d = join(d, "..");
+  }
+  return "";
+}
+
+}  // namespace
+
+double Eos::computeDensity(double pressure, double temperature) const {
+  if (type == ECG_EOS_SG) return (pressure + pInf) / std::max(((gamma - 1.) * cv * temperature), epsilon);
+  return pressure / std::max(((gamma - 1.) * cv * temperature), epsilon);
+}
+
+ecg_eos Eos::toAbi() const {
+  ecg_eos e{};
+  e.type = type;
+  e.gamma = gamma;
+  e.pinf = pInf;
+  e.cv = cv;
+  e.eref = energyRef;
+  return e;
+}
+
+Limiter Limiter::fromName(const std::string& xmlName, bool volumeFraction) {
+  Limiter l;
+  const std::string u = upper(xmlName);
+  if (u == "MINMOD") l.kind = ECG_LIM_MINMOD;
+  else if (u == "VANLEER") l.kind = ECG_LIM_VANLEER;
+  else if (u == "VANALBADA") l.kind = ECG_LIM_VANALBADA;
+  else if (u == "SUPERBEE") l.kind = ECG_LIM_SUPERBEE;
+  else if (u == "MC") l.kind = ECG_LIM_MC;
+  else if (u == "THINC") {
+    if (!volumeFraction) throw InputError(ECG_E_ARG, "THINC can only be a volume-fraction limiter");
+    throw InputError(ECG_E_UNSUPPORTED, "the THINC volume-fraction limiter is not built");
+  } else {
+    throw InputError(ECG_E_ARG, "unknown limiter " + xmlName);
+  }
+  return l;
+}
+
+int Model::abiModel() const {
+  if (name == "EULER") return ECG_EULER;
+  if (name == "PRESSUREVELOCITYEQ") return ECG_PUEQ;
+  return ECG_UEQ;
+}
+
+namespace {
+
+void read_main(const std::string& file, CaseDescription& c) {
+  auto doc = load_xml(file);
+  if (doc->name != "computationParam") bad(file, "root element must be <computationParam>");
+  const XmlElement& run = need(doc->child("run"), "run", file);
+  c.control.runName = run.trimmed_text();
+  const XmlElement& tc = need(doc->child("timeControlMode"), "timeControlMode", file);
+  c.control.iterationsMode = boolean(tc, "iterations", file);
+  if (c.control.iterationsMode) {
+    const XmlElement& it = need(tc.child("iterations"), "iterations", file);
+    c.control.iterations = integer(it, "number", file);
+    c.control.iterFreq = (int)integer(it, "iterFreq", file);
+  } else {
+    const XmlElement& pt = need(tc.child("physicalTime"), "physicalTime", file);
+    c.control.totalTime = num(pt, "totalTime", file);
+    c.control.timeFreq = num(pt, "timeFreq", file);
+  }
+  c.control.cfl = num(need(doc->child("computationControl"), "computationControl", file), "CFL", file);
+  const XmlElement* so = doc->child("secondOrder");
+  if (!so) {
+    c.order = "FIRSTORDER";
+    for (auto& l : c.limiters) l.kind = ECG_LIM_NONE;
+  } else {
+    c.order = "SECONDORDER";
+    const XmlElement& g = need(so->child("globalLimiter"), "globalLimiter", file);
+    const std::string global = g.trimmed_text();
+    if (global.empty()) bad(file, "empty <globalLimiter>");
+    std::string iface = global, gvf = global;
+    if (const XmlElement* e = so->child("interfaceLimiter")) iface = e->trimmed_text();
+    if (const XmlElement* e = so->child("globalVolumeFractionLimiter")) gvf = e->trimmed_text();
+    std::string ivf = iface;
+    if (const XmlElement* e = so->child("interfaceVolumeFractionLimiter")) ivf = e->trimmed_text();
+    c.limiters[0] = Limiter::fromName(global, false);
+    c.limiters[1] = Limiter::fromName(iface, false);
+    c.limiters[2] = Limiter::fromName(gvf, true);
+    c.limiters[3] = Limiter::fromName(ivf, true);
+  }
+  if (const XmlElement* r = doc->child("restartSimulation")) {
+    if (has(*r, "restartFileNumber") && integer(*r, "restartFileNumber", file) != 0)
+      unsupported(file, "restarting from an output file is not built");
+  }
+}
+
+void read_mesh(const std::string& file, CaseDescription& c) {
+  auto doc = load_xml(file);
+  if (doc->name != "mesh") bad(file, "root element must be <mesh>");
+  const XmlElement& type = need(doc->child("type"), "type", file);
+  const std::string* st = type.attr("structure");
+  if (!st || upper(*st) != "CARTESIAN") unsupported(file, "only Cartesian meshes are built");
+  const XmlElement& cm = need(doc->child("cartesianMesh"), "cartesianMesh", file);
+  const XmlElement& dim = need(cm.child("dimensions"), "dimensions", file);
+  const XmlElement& nc = need(cm.child("numberCells"), "numberCells", file);
+  c.config.lx = num(dim, "x", file);
+  c.config.ly = num(dim, "y", file);
+  c.config.lz = num(dim, "z", file);
+  c.config.nx = (int)integer(nc, "x", file);
+  c.config.ny = (int)integer(nc, "y", file);
+  c.config.nz = (int)integer(nc, "z", file);
+  if (c.config.nx <= 1) bad(file, "Number of cells in the x-direction has to be superior to 1");
+  if (c.config.nz > 1 && c.config.ny <= 1)
+    bad(file, "Number of cells in the z-direction can't be superior to 1 if number of cells in the y-direction is equal to 1");
+  if (cm.child("meshStretching")) unsupported(file, "mesh stretching is not built");
+  if (const XmlElement* amr = cm.child("AMR")) {
+    if (integer(*amr, "lvlMax", file) != 0) unsupported(file, "adaptive mesh refinement is not built (lvlMax must be 0)");
+  }
+}
+
+Eos read_eos(const std::string& dir, const std::string& name) {
+  const std::string file = join(dir, name);
+  auto doc = load_xml(file);
+  if (doc->name != "parametersEOS") bad(file, "root element must be <parametersEOS>");
+  const XmlElement& e = need(doc->child("EOS"), "EOS", file);
+  const std::string* t = e.attr("type");
+  if (!t) bad(file, "<EOS> needs attribute type");
+  Eos eos;
+  eos.name = name;
+  const std::string u = upper(*t);
+  const XmlElement& p = need(doc->child("parameters"), "parameters", file);
+  if (u == "SG") {
+    eos.type = ECG_EOS_SG;
+    eos.pInf = num(p, "pInf", file);
+  } else if (u == "IG") {
+    eos.type = ECG_EOS_IG;
+  } else {
+    unsupported(file, "EOS type " + *t + " is not built (IG and SG are)");
+  }
+  eos.gamma = num(p, "gamma", file);
+  eos.cv = num(p, "cv", file);
+  eos.energyRef = num(p, "energyRef", file);
+  eos.entropyRef = num(p, "entropyRef", file);
+  return eos;
+}
+
+void read_model(const std::string& file, const std::string& eosDir, CaseDescription& c) {
+  auto doc = load_xml(file);
+  if (doc->name != "model") bad(file, "root element must be <model>");
+  const XmlElement& fm = need(doc->child("flowModel"), "flowModel", file);
+  const std::string* nm = fm.attr("name");
+  if (!nm) bad(file, "<flowModel> needs attribute name");
+  c.model.name = upper(*nm);
+  if (has(fm, "numberTransports") && integer(fm, "numberTransports", file) != 0)
+    unsupported(file, "transported scalars are not built");
+  if (c.model.name == "EULER") {
+    c.model.numberPhases = 1;
+  } else if (c.model.name == "PRESSUREVELOCITYEQ" || c.model.name == "VELOCITYEQ") {
+    c.model.numberPhases = (int)integer(fm, "numberPhases", file);
+    c.model.alphaNull = boolean(fm, "alphaNull", file);
+  } else {
+    unsupported(file, "flow model " + *nm + " is not built (EULER, PRESSUREVELOCITYEQ and VELOCITYEQ are)");
+  }
+  if (const XmlElement* lm = doc->child("lowMach"))
+    if (boolean(*lm, "state", file)) unsupported(file, "low-Mach preconditioning is not built");
+  for (const char* n : {"additionalPhysic", "sourceTerms", "symmetryTerm", "transport"})
+    if (doc->child(n)) unsupported(file, std::string("<") + n + "> is not built");
+  const std::string* g = nullptr;
+  if (const XmlElement* geo = doc->child("geometry"))
+    if ((g = geo->attr("smoothCrossSection1d")) && upper(*g) == "TRUE") unsupported(file, "1D cross sections are not built");
+  std::string dir = eosDir.empty() ? find_eos_dir(file.substr(0, file.find_last_of('/') + 1)) : eosDir;
+  auto eosEls = doc->children_named("EOS");
+  if (eosEls.empty()) bad(file, "no <EOS> element");
+  if ((int)eosEls.size() != c.model.numberPhases) bad(file, "one <EOS> per phase is required");
+  if (dir.empty()) bad(file, "no libEOS directory found for the EOS files");
+  for (const XmlElement* e : eosEls) {
+    const std::string* n = e->attr("name");
+    if (!n || n->empty()) bad(file, "<EOS> needs attribute name");
+    c.eos.push_back(read_eos(dir, *n));
+  }
+  const double eps = c.model.alphaNull ? 1.e-15 : 0.;
+  for (auto& e : c.eos) e.epsilon = eps;
+  for (const XmlElement* r : doc->children_named("relaxation")) {
+    const std::string* t = r->attr("type");
+    if (!t) bad(file, "<relaxation> needs attribute type");
+    if (upper(*t) != "P") unsupported(file, "relaxation " + *t + " is not built (infinite pressure relaxation is)");
+    const std::string* sp = r->attr("speed");
+    if (sp && upper(*sp) == "FINITE") unsupported(file, "finite-rate pressure relaxation is not built");
+    if (!c.model.relaxations.empty()) bad(file, "pressure relaxation given twice");
+    c.model.relaxations.push_back(Relaxation());
+  }
+}
+
+void read_state(const XmlElement& st, const CaseDescription& c, const std::string& file, ecg_domain& d) {
+  const int N = c.model.numberPhases;
+  auto mats = st.children_named("material");
+  if ((int)mats.size() != N) bad(file, "state " + (st.attr("name") ? *st.attr("name") : std::string("?")) +
+                                           " needs one material per phase");
+  double pmix = 0.;
+  const XmlElement* mix = st.child("mixture");
+  if (c.model.name != "EULER") {
+    if (!mix) bad(file, "state needs a <mixture>");
+    const XmlElement& v = need(mix->child("velocity"), "velocity", file);
+    d.vel[0] = num(v, "x", file); d.vel[1] = num(v, "y", file); d.vel[2] = num(v, "z", file);
+    if (c.model.name == "PRESSUREVELOCITYEQ") pmix = num(need(mix->child("dataMix"), "dataMix", file), "pressure", file);
+  }
+  for (int k = 0; k < N; k++) {
+    const XmlElement& m = *mats[k];
+    const std::string* type = m.attr("type");
+    if (!type || upper(*type) != "FLUID") unsupported(file, "only fluid materials are built");
+    const std::string* en = m.attr("EOS");
+    if (!en || *en != c.eos[k].name) bad(file, "Materials in used states should be ordered in reference to model.xml");
+    const XmlElement& df = need(m.child("dataFluid"), "dataFluid", file);
+    const Eos& eos = c.eos[k];
+    if (c.model.name == "PRESSUREVELOCITYEQ") {
+      d.alpha[k] = num(df, "alpha", file);
+      const bool hr = has(df, "density"), ht = has(df, "temperature");
+      if (hr && ht) bad(file, "only one of following is required : density, temperature");
+      if (!hr && !ht) bad(file, "density or temperature is required");
+      d.p[k] = pmix;
+      d.rho[k] = hr ? num(df, "density", file) : eos.computeDensity(pmix, num(df, "temperature", file));
+    } else {
+      d.alpha[k] = (c.model.name == "EULER") ? 1. : num(df, "alpha", file);
+      const bool hr = has(df, "density"), hp = has(df, "pressure"), ht = has(df, "temperature");
+      if ((int)hr + (int)hp + (int)ht != 2) bad(file, "only two of following is required : density, pressure, temperature");
+      if (hr && ht) bad(file, "impossible to initialize a phase with density and temperature");
+      d.p[k] = num(df, "pressure", file);
+      d.rho[k] = hr ? num(df, "density", file) : eos.computeDensity(d.p[k], num(df, "temperature", file));
+      if (c.model.name == "EULER") {
+        const XmlElement& v = need(df.child("velocity"), "velocity", file);
+        d.vel[0] = num(v, "x", file); d.vel[1] = num(v, "y", file); d.vel[2] = num(v, "z", file);
+      }
+    }
+  }
+}
+
+void read_initial_conditions(const std::string& file, CaseDescription& c) {
+  auto doc = load_xml(file);
+  if (doc->name != "CI") bad(file, "root element must be <CI>");
+  const XmlElement& pd = need(doc->child("physicalDomains"), "physicalDomains", file);
+  auto states = doc->children_named("state");
+  for (const XmlElement* de : pd.children_named("domain")) {
+    const std::string* name = de->attr("name");
+    const std::string* state = de->attr("state");
+    const std::string* type = de->attr("type");
+    if (!name || !state || !type) bad(file, "<domain> needs name, state and type");
+    if (has(*de, "physicalEntity") && integer(*de, "physicalEntity", file) != 0)
+      unsupported(file, "physical entities / immersed boundaries are not built");
+    const XmlElement* st = nullptr;
+    for (const XmlElement* s : states)
+      if (s->attr("name") && upper(*s->attr("name")) == upper(*state)) { st = s; break; }
+    if (!st) bad(file, "state " + *state + " not found");
+    ecg_domain d{};
+    read_state(*st, c, file, d);
+    const std::string t = upper(*type);
+    if (t == "ENTIREDOMAIN") {
+      d.kind = ECG_DOM_ENTIRE;
+    } else if (t == "HALFSPACE") {
+      const XmlElement& h = need(de->child("dataHalfSpace"), "dataHalfSpace", file);
+      d.kind = ECG_DOM_HALFSPACE;
+      d.origin = num(h, "origin", file);
+      d.axis = axis_of(h.attr("axis") ? *h.attr("axis") : "", file);
+      const std::string dir = h.attr("direction") ? upper(*h.attr("direction")) : "";
+      if (dir == "POSITIVE") d.dir = 1;
+      else if (dir == "NEGATIVE") d.dir = -1;
+      else bad(file, "direction must be positive or negative");
+    } else if (t == "DISC") {
+      const XmlElement& h = need(de->child("dataDisc"), "dataDisc", file);
+      d.kind = ECG_DOM_DISC;
+      d.radius = num(h, "radius", file);
+      d.axis = axis_of(h.attr("axis1") ? *h.attr("axis1") : "", file);
+      d.axis2 = axis_of(h.attr("axis2") ? *h.attr("axis2") : "", file);
+      const XmlElement& ce = need(h.child("center"), "center", file);
+      d.center[0] = num(ce, "x", file); d.center[1] = num(ce, "y", file); d.center[2] = num(ce, "z", file);
+    } else if (t == "SPHERE") {
+      const XmlElement& h = need(de->child("dataSphere"), "dataSphere", file);
+      d.kind = ECG_DOM_SPHERE;
+      d.radius = num(h, "radius", file);
+      const XmlElement& ce = need(h.child("center"), "center", file);
+      d.center[0] = num(ce, "x", file); d.center[1] = num(ce, "y", file); d.center[2] = num(ce, "z", file);
+    } else if (t == "RECTANGLE") {
+      const XmlElement& h = need(de->child("dataRectangle"), "dataRectangle", file);
+      d.kind = ECG_DOM_RECTANGLE;
+      d.axis = axis_of(h.attr("axis1") ? *h.attr("axis1") : "", file);
+      d.axis2 = axis_of(h.attr("axis2") ? *h.attr("axis2") : "", file);
+      d.len[d.axis] = num(h, "lAxis1", file);
+      d.len[d.axis2] = num(h, "lAxis2", file);
+      const XmlElement& v = need(h.child("posInferiorVertex"), "posInferiorVertex", file);
+      d.lo[0] = num(v, "x", file); d.lo[1] = num(v, "y", file); d.lo[2] = num(v, "z", file);
+    } else if (t == "CUBOID") {
+      const XmlElement& h = need(de->child("dataCuboid"), "dataCuboid", file);
+      d.kind = ECG_DOM_CUBOID;
+      d.len[0] = num(h, "lAxisX", file); d.len[1] = num(h, "lAxisY", file); d.len[2] = num(h, "lAxisZ", file);
+      const XmlElement& v = need(h.child("posInferiorVertex"), "posInferiorVertex", file);
+      d.lo[0] = num(v, "x", file); d.lo[1] = num(v, "y", file); d.lo[2] = num(v, "z", file);
+    } else {
+      unsupported(file, "domain type " + *type + " is not built");
+    }
+    c.domains.push_back(d);
+    c.domainNames.push_back(*name);
+  }
+  if (c.domains.empty()) bad(file, "no <domain>");
+  // Boundary conditions are assigned to X-, X+, Y-, Y+, Z-, Z+ in document
+  // order; sides not given stay non-reflecting (MeshCartesian.cpp:86-125).
+  const XmlElement& bcs = need(doc->child("boundaryConditions"), "boundaryConditions", file);
+  int i = 0;
+  for (const XmlElement* b : bcs.children_named("boundCond")) {
+    if (i >= 6) bad(file, "more than six boundary conditions");
+    const std::string* t = b->attr("type");
+    if (!t) bad(file, "<boundCond> needs attribute type");
+    const std::string u = upper(*t);
+    if (u == "NONREFLECTING") c.config.bc[i] = ECG_BC_NONREFLECTING;
+    else if (u == "SYMMETRY" || u == "WALL") c.config.bc[i] = ECG_BC_WALL;
+    else unsupported(file, "boundary condition " + *t + " is not built");
+    i++;
+  }
+  for (; i < 6; i++) c.config.bc[i] = ECG_BC_NONREFLECTING;
+}
+
+}  // namespace
+
+CaseDescription Input::readInputXML(const std::string& caseDir, const std::string& eosDir, int precision) {
+  CaseDescription c;
+  read_main(join(caseDir, "main.xml"), c);
+  read_mesh(join(caseDir, "mesh.xml"), c);
+  read_model(join(caseDir, "model.xml"), eosDir, c);
+  read_initial_conditions(join(caseDir, "initialConditions.xml"), c);
+  ecg_config& g = c.config;
+  g.model = c.model.abiModel();
+  g.nph = c.model.numberPhases;
+  if (g.nph > ECG_MAX_PHASES) throw InputError(ECG_E_UNSUPPORTED, "more than four phases");
+  for (int k = 0; k < g.nph; k++) g.eos[k] = c.eos[k].toAbi();
+  g.order = (c.order == "SECONDORDER") ? 2 : 1;
+  for (int q = 0; q < 4; q++) g.lim[q] = c.limiters[q].kind;
+  g.alpha_null = c.model.alphaNull ? 1 : 0;
+  g.relax_p = (g.model == ECG_PUEQ || !c.model.relaxations.empty()) ? 1 : 0;
+  g.cfl = c.control.cfl;
+  g.precision = precision;
+  return c;
+}
+
+Run::Run(const CaseDescription& c) : m_case(c) {}
+
+Run::~Run() {
+  if (m_ctx) ecg_destroy(m_ctx);
+}
+
+void Run::initialize() {
+  m_ctx = ecg_create(&m_case.config);
+  if (!m_ctx) throw InputError(ECG_E_ARG, ecg_last_error());
+  const int rc = ecg_fill(m_ctx, m_case.domains.data(), (int)m_case.domains.size());
+  if (rc != ECG_OK) throw InputError(rc, ecg_last_error());
+}
+
+ecg_step_stats Run::solver() {
+  ecg_step_stats st{};
+  const RunControl& t = m_case.control;
+  const int rc = t.iterationsMode ? ecg_run(m_ctx, t.iterations, 0., &st) : ecg_run(m_ctx, 0, t.totalTime, &st);
+  if (rc < 0) throw InputError(rc, ecg_last_error());
+  return st;
+}
+
+std::vector<std::vector<double>> Run::state() const {
+  const int nv = ecg_nvar(m_ctx);
+  const int64_t n = ecg_ncell(m_ctx);
+  std::vector<std::vector<double>> out(nv, std::vector<double>((size_t)n));
+  std::vector<double*> ptr(nv);
+  for (int v = 0; v < nv; v++) ptr[v] = out[v].data();
+  const int rc = ecg_download_state(m_ctx, ptr.data(), nv);
+  if (rc != ECG_OK) throw InputError(rc, ecg_last_error());
+  return out;
+}
+
+}  // namespace ecg_host
+
+extern "C" int ecg_read_case(const char* case_dir, const char* eos_dir, int precision, ecg_config* cfg,
+                             ecg_domain* doms, int max_doms, int* ndoms, ecg_run_control* rc) {
+  if (!case_dir || !cfg || !ndoms) return ecg::api_fail(ECG_E_ARG, "null argument");
+  try {
+    const ecg_host::CaseDescription c =
+        ecg_host::Input::readInputXML(case_dir, eos_dir ? eos_dir : "", precision);
+    *ndoms = (int)c.domains.size();
+    if ((int)c.domains.size() > max_doms || (!doms && !c.domains.empty()))
+      return ecg::api_fail(ECG_E_ARG, "the case has " + std::to_string(c.domains.size()) + " domains, more than max_doms");
+    *cfg = c.config;
+    std::copy(c.domains.begin(), c.domains.end(), doms);
+    if (rc) {
+      std::memset(rc, 0, sizeof(*rc));
+      std::strncpy(rc->run_name, c.control.runName.c_str(), sizeof(rc->run_name) - 1);
+      rc->iterations_mode = c.control.iterationsMode ? 1 : 0;
+      rc->iterations = c.control.iterations;
+      rc->iter_freq = c.control.iterFreq;
+      rc->total_time = c.control.totalTime;
+      rc->time_freq = c.control.timeFreq;
+    }
+    return ECG_OK;
+  } catch (const ecg_host::InputError& e) {
+    return ecg::api_fail(e.code, e.what());
+  } catch (const ecg_host::XmlError& e) {
+    return ecg::api_fail(ECG_E_ARG, e.what());
+  } catch (const std::exception& e) {
+    return ecg::api_fail(ECG_E_ARG, e.what());
+  }
+}

@@ -1,0 +1,112 @@
+// Host-side C++ layer of the drop-in: the reference's case inputs (main.xml,
+// mesh.xml, model.xml, initialConditions.xml and the libEOS files) read into
+// the C ABI's ecg_config / ecg_domain, and a Run object that drives the
+// device-resident time loop.  Class names follow the reference (Eos, Limiter,
+// Relaxation, Model, Input, Run); each holds only what the explicit Cartesian
+// step needs and forwards the work to libecogen_amd's C ABI.
+#pragma once
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ecogen_amd.h"
+
+namespace ecg_host {
+
+// Input problems (the reference's ErrorXML* family, src/Errors.h:113-644).
+struct InputError : std::runtime_error {
+  int code;
+  InputError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Eos::assignParametersEos (EosSG.cpp:48-57, EosIG.cpp:47-55) and the two
+// state conversions the initial conditions use.
+class Eos {
+ public:
+  std::string name;       // file name, the key materials refer to
+  int type = ECG_EOS_IG;  // ecg_eos_type
+  double gamma = 0., pInf = 0., cv = 0., energyRef = 0., entropyRef = 0.;
+  double epsilon = 0.;    // epsilonAlphaNull (Eos.cpp:88-97)
+
+  // EosSG::computeDensity / EosIG::computeDensity (EosSG.cpp:87-90, EosIG.cpp:85-88).
+  double computeDensity(double pressure, double temperature) const;
+  ecg_eos toAbi() const;
+};
+
+// Limiter kinds by their XML names (Input.cpp:228-303).
+class Limiter {
+ public:
+  int kind = ECG_LIM_NONE;
+  static Limiter fromName(const std::string& xmlName, bool volumeFraction);
+};
+
+// <relaxation type="P" speed="infinite"/> (Input.cpp:866-886).
+class Relaxation {
+ public:
+  std::string type = "P";
+  bool infinite = true;
+};
+
+// Flow model of model.xml (Input.cpp:564-913).
+class Model {
+ public:
+  std::string name;       // EULER, PRESSUREVELOCITYEQ, VELOCITYEQ
+  int numberPhases = 1;
+  bool alphaNull = false;
+  std::vector<Relaxation> relaxations;
+  int abiModel() const;
+  const std::string& whoAmI() const { return name; }
+};
+
+// Time control of main.xml.
+struct RunControl {
+  std::string runName;
+  bool iterationsMode = false;
+  long long iterations = 0;
+  int iterFreq = 0;
+  double totalTime = 0., timeFreq = 0.;
+  double cfl = 0.;
+};
+
+// What the reference's Input::readInputXML assembles for the hot path.
+struct CaseDescription {
+  RunControl control;
+  Model model;
+  std::vector<Eos> eos;
+  Limiter limiters[4];    // global, interface, global volume fraction, interface volume fraction
+  std::string order;      // FIRSTORDER / SECONDORDER
+  ecg_config config{};
+  std::vector<ecg_domain> domains;
+  std::vector<std::string> domainNames;
+};
+
+// Input::readInputXML.  caseDir holds the four case files; eosDir the EOS
+// files (empty: the first libEOS directory found in caseDir or its parents, as
+// the reference looks in <work folder>/libEOS).
+class Input {
+ public:
+  static CaseDescription readInputXML(const std::string& caseDir, const std::string& eosDir, int precision);
+};
+
+// Run: owns the device context of one case and the reference's time loop
+// (Run::initialize / Run::solver / Run::finalize, Run.cpp:82-498).
+class Run {
+ public:
+  explicit Run(const CaseDescription& c);
+  ~Run();
+  Run(const Run&) = delete;
+  Run& operator=(const Run&) = delete;
+  void initialize();
+  ecg_step_stats solver();
+  std::vector<std::vector<double>> state() const;
+  ecg_ctx* context() const { return m_ctx; }
+  const CaseDescription& description() const { return m_case; }
+
+ private:
+  CaseDescription m_case;
+  ecg_ctx* m_ctx = nullptr;
+};
+
+}  // namespace ecg_host

@@ -110,10 +110,29 @@ typedef struct {
 
 typedef struct ecg_ctx ecg_ctx;
 
+/* Time control of a case's main.xml (Input.cpp:179-203). */
+typedef struct {
+  char run_name[128];
+  int iterations_mode;    /* 1: stop after `iterations` steps; 0: at physical time total_time */
+  int64_t iterations;
+  int iter_freq;
+  double total_time, time_freq;
+} ecg_run_control;
+
 /* Library / device ------------------------------------------------------- */
 const char* ecg_last_error(void);
 const char* ecg_version(void);
 int ecg_device_count(void);
+
+/* Input::readInputXML (src/InputOutput/Input.cpp:54-73) for the hot path: reads
+ * case_dir/{main,mesh,model,initialConditions}.xml and the EOS files of eos_dir
+ * (NULL: the first libEOS directory in case_dir or its parents) into *cfg (with
+ * the given precision form), doms[0..*ndoms) and *rc (may be NULL).  Features the
+ * library does not build (AMR, stretching, other models/EOS/limiters/boundaries,
+ * transports, additional physics) return ECG_E_UNSUPPORTED with the reason; when
+ * max_doms is too small, *ndoms holds the count needed and ECG_E_ARG is returned. */
+int ecg_read_case(const char* case_dir, const char* eos_dir, int precision, ecg_config* cfg, ecg_domain* doms,
+                  int max_doms, int* ndoms, ecg_run_control* rc);
 
 /* Context lifetime (Run::Run / Run::initialize / Run::finalize). */
 ecg_ctx* ecg_create(const ecg_config* cfg);

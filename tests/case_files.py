@@ -1,0 +1,181 @@
+"""Writers of case directories in the reference's XML input format (main.xml,
+mesh.xml, model.xml, initialConditions.xml and a libEOS folder), used by the
+case-reader tests.  The element and attribute names are those the reference's
+Input.cpp reads (Input.cpp:77-1193, GD*.cpp, Phase*/Mix* constructors)."""
+import os
+
+IG_AIR_XML = """<?xml version = "1.0" encoding = "UTF-8" standalone = "yes"?>
+<parametersEOS>
+  <EOS type="IG"/>
+  <parameters gamma="1.4" cv="800" energyRef="0." entropyRef="0."/>
+</parametersEOS>
+"""
+SG_WATER_XML = """<?xml version = "1.0" encoding = "UTF-8" standalone = "yes"?>
+<parametersEOS>
+  <EOS type="SG"/>
+  <!-- stiffened gas -->
+  <parameters gamma="4.4" pInf="6.e8" cv="1000.0" energyRef="0." entropyRef="0."/>
+</parametersEOS>
+"""
+
+
+def _write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def write_eos(root):
+    os.makedirs(os.path.join(root, "libEOS"), exist_ok=True)
+    _write(os.path.join(root, "libEOS", "IG_air.xml"), IG_AIR_XML)
+    _write(os.path.join(root, "libEOS", "SG_water.xml"), SG_WATER_XML)
+
+
+def write_sod(case_dir, n=200, iterations=None, t_end=5.e-4):
+    """1D shock tube (Euler, ideal gas): (rho, p) = (1, 1e5) | (0.125, 1e4) at x = 0.5."""
+    os.makedirs(case_dir, exist_ok=True)
+    tc = (f'<timeControlMode iterations="true">\n    <iterations number="{iterations}" iterFreq="{iterations}"/>\n'
+          f'    <physicalTime totalTime="1." timeFreq="1."/>\n  </timeControlMode>' if iterations else
+          f'<timeControlMode iterations="false">\n    <iterations number="10" iterFreq="1"/>\n'
+          f'    <physicalTime totalTime="{t_end!r}" timeFreq="{t_end!r}"/>\n  </timeControlMode>')
+    _write(os.path.join(case_dir, "main.xml"), f"""<?xml version = "1.0" encoding = "UTF-8" standalone = "yes"?>
+<computationParam>
+  <run>sodTube</run>
+  <outputMode format="XML" binary="false" precision="10"/>
+  {tc}
+  <computationControl CFL="0.8"/>
+</computationParam>
+""")
+    _write(os.path.join(case_dir, "mesh.xml"), f"""<?xml version = "1.0" encoding = "UTF-8" standalone = "yes"?>
+<mesh>
+  <type structure="cartesian"/>
+  <cartesianMesh>
+    <dimensions x="1." y="0.1" z="0.1"/>
+    <numberCells x="{n}" y="1" z="1"/>
+  </cartesianMesh>
+</mesh>
+""")
+    _write(os.path.join(case_dir, "model.xml"), """<?xml version='1.0' encoding='UTF-8' standalone='yes'?>
+<model>
+  <flowModel name="Euler"/>
+  <EOS name="IG_air.xml"/>
+</model>
+""")
+    _write(os.path.join(case_dir, "initialConditions.xml"), """<?xml version = "1.0" encoding = "UTF-8" standalone = "yes"?>
+<CI>
+  <physicalDomains>
+    <domain name="base" state="lowPressure" type="entireDomain"/>
+    <domain name="HP" state="highPressure" type="halfSpace">
+      <dataHalfSpace axis="x" origin="0.5" direction="negative"/>
+    </domain>
+  </physicalDomains>
+  <boundaryConditions>
+    <boundCond name="CLXm" type="nonReflecting" number="1"/>
+    <boundCond name="CLXp" type="nonReflecting" number="2"/>
+  </boundaryConditions>
+  <state name="lowPressure">
+    <material type="fluid" EOS="IG_air.xml">
+      <dataFluid density="0.125" pressure="1.e4">
+        <velocity x="0." y="0." z="0."/>
+      </dataFluid>
+    </material>
+  </state>
+  <state name="highPressure">
+    <material type="fluid" EOS="IG_air.xml">
+      <dataFluid density="1." pressure="1.e5">
+        <velocity x="0." y="0." z="0."/>
+      </dataFluid>
+    </material>
+  </state>
+</CI>
+""")
+
+
+def write_kapila_bubble(case_dir, n=48, iterations=12, second_order=True, model="PressureVelocityEq"):
+    """2D water/air: an air disc and a water rectangle in air-dominated water, a
+    shocked half-space, wall on Y-, temperature-given density for the shocked air."""
+    os.makedirs(case_dir, exist_ok=True)
+    so = ("""  <secondOrder>
+    <globalLimiter>minmod</globalLimiter>
+    <interfaceLimiter>superbee</interfaceLimiter>
+    <globalVolumeFractionLimiter>vanleer</globalVolumeFractionLimiter>
+  </secondOrder>
+""" if second_order else "")
+    _write(os.path.join(case_dir, "main.xml"), f"""<?xml version = "1.0" encoding = "UTF-8" standalone = "yes"?>
+<computationParam>
+  <run>bubble</run>
+  <outputMode format="GNU" binary="false"/>
+  <timeControlMode iterations="true">
+    <iterations number="{iterations}" iterFreq="{iterations}"/>
+    <physicalTime totalTime="1." timeFreq="1."/>
+  </timeControlMode>
+  <computationControl CFL="0.8"/>
+{so}</computationParam>
+""")
+    _write(os.path.join(case_dir, "mesh.xml"), f"""<?xml version = "1.0" encoding = "UTF-8" standalone = "yes"?>
+<mesh>
+  <type structure="cartesian"/>
+  <cartesianMesh>
+    <dimensions x="1." y="0.75" z="1."/>
+    <numberCells x="{n}" y="{3 * n // 4}" z="1"/>
+    <AMR lvlMax="0" criteriaVar="0.1" varRho="true" varP="true" varU="false" varAlpha="true" xiSplit="0.1" xiJoin="0.1"/>
+  </cartesianMesh>
+</mesh>
+""")
+    relax = '\n  <relaxation type="P" speed="infinite"/>' if model == "VelocityEq" else ""
+    _write(os.path.join(case_dir, "model.xml"), f"""<?xml version='1.0' encoding='UTF-8' standalone='yes'?>
+<model>
+  <flowModel name="{model}" numberPhases="2" alphaNull="false"/>
+  <EOS name="SG_water.xml"/>
+  <EOS name="IG_air.xml"/>{relax}
+</model>
+""")
+    pueq = model == "PressureVelocityEq"
+
+    def state(name, aw, rw, p, u, air_temp=None):
+        air = (f'<dataFluid alpha="{1 - aw!r}" temperature="{air_temp!r}"/>' if air_temp else
+               f'<dataFluid alpha="{1 - aw!r}" density="1.2"/>')
+        if pueq:
+            water = f'<dataFluid alpha="{aw!r}" density="{rw!r}"/>'
+            mix = f'<dataMix pressure="{p!r}"/>\n      '
+        else:
+            water = f'<dataFluid alpha="{aw!r}" density="{rw!r}" pressure="{p!r}"/>'
+            air = (f'<dataFluid alpha="{1 - aw!r}" pressure="{p!r}" temperature="{air_temp!r}"/>' if air_temp else
+                   f'<dataFluid alpha="{1 - aw!r}" density="1.2" pressure="{p!r}"/>')
+            mix = ""
+        return f"""  <state name="{name}">
+    <material type="fluid" EOS="SG_water.xml">
+      {water}
+    </material>
+    <material type="fluid" EOS="IG_air.xml">
+      {air}
+    </material>
+    <mixture>
+      {mix}<velocity x="{u!r}" y="0." z="0."/>
+    </mixture>
+  </state>
+"""
+    _write(os.path.join(case_dir, "initialConditions.xml"), f"""<?xml version = "1.0" encoding = "UTF-8" standalone = "yes"?>
+<CI>
+  <physicalDomains>
+    <domain name="base" state="water" type="entireDomain"/>
+    <domain name="bubble" state="air" type="disc">
+      <dataDisc axis1="x" axis2="y" radius="0.15">
+        <center x="0.55" y="0.35" z="0."/>
+      </dataDisc>
+    </domain>
+    <domain name="block" state="air" type="rectangle">
+      <dataRectangle axis1="x" axis2="y" lAxis1="0.1" lAxis2="0.2">
+        <posInferiorVertex x="0.75" y="0.05" z="0."/>
+      </dataRectangle>
+    </domain>
+    <domain name="shock" state="shocked" type="halfSpace">
+      <dataHalfSpace axis="x" origin="0.2" direction="negative"/>
+    </domain>
+  </physicalDomains>
+  <boundaryConditions>
+    <boundCond name="CLXm" type="nonReflecting" number="1"/>
+    <boundCond name="CLXp" type="nonReflecting" number="2"/>
+    <boundCond name="CLYm" type="{'nonReflecting' if second_order else 'wall'}" number="3"/>
+  </boundaryConditions>
+{state("water", 1 - 1e-6, 1000.0, 1.e5, 0.0)}{state("air", 1e-6, 1000.0, 1.e5, 0.0)}{state("shocked", 1 - 1e-6, 1100.0, 3.e8, 120.0, air_temp=400.0)}</CI>
+""")
