@@ -48,9 +48,20 @@ void tile_extent(int dim, int t[3]) {
 }
 
 // Run::solver bookkeeping after a step (Run.cpp:417-486): t += dt, dt = cfl*dtMax.
-__global__ void k_clock(Clock* clk, double cfl) {
-  if (threadIdx.x != 0 || clk->done) return;
-  const double dtmax = __longlong_as_double((long long)clk->dtmax_bits);
+// One wave: lane l reduces and resets counter slot l.
+__global__ void k_clock(Clock* clk, Counters* cnt, double cfl) {
+  const int l = threadIdx.x;
+  unsigned long long b = (unsigned long long)__double_as_longlong(1.e10);
+  if (l < ATOMIC_SLOTS) b = cnt[l].dtmax_bits;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long q = __shfl_xor(b, o, 64);
+    b = (q < b) ? q : b;
+  }
+  if (clk->done) return;
+  if (l < ATOMIC_SLOTS) cnt[l].dtmax_bits = (uint64_t)__double_as_longlong(1.e10);
+  if (l != 0) return;
+  const double dtmax = __longlong_as_double((long long)b);
   clk->dt_max = dtmax;
   clk->t += clk->dt;
   clk->iteration += 1;
@@ -61,12 +72,12 @@ __global__ void k_clock(Clock* clk, double cfl) {
     clk->done = 1;
   }
   clk->dt = clk->dt_next;
-  clk->dtmax_bits = (uint64_t)__double_as_longlong(1.e10);
 }
 
-// Host-set clock: reset or explicit-dt steps.
-__global__ void k_clock_set(Clock* clk, Clock v) {
+// Host-set clock: reset or explicit-dt steps; the dtMax slots restart at 1e10.
+__global__ void k_clock_set(Clock* clk, Counters* cnt, Clock v) {
   if (threadIdx.x == 0) *clk = v;
+  if (threadIdx.x < ATOMIC_SLOTS) cnt[threadIdx.x].dtmax_bits = (uint64_t)__double_as_longlong(1.e10);
 }
 
 // Plain streaming copy (16 bytes per lane per iteration), used to calibrate byte counters.
@@ -126,7 +137,9 @@ __global__ __launch_bounds__(256) void k_state_diff(const SA* __restrict__ a, co
   }
 }
 
-void launch_clock(Clock* clk, double cfl, hipStream_t s) { hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, clk, cfl); }
+void launch_clock(Clock* clk, Counters* cnt, double cfl, hipStream_t s) {
+  hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, s, clk, cnt, cfl);
+}
 
 }  // namespace ecg
 
@@ -364,9 +377,9 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   if (ok) chk(hipMalloc(&c->tables, sizeof(double) * tab.size()), "hipMalloc(tables)");
   if (ok) chk(hipMemcpy(c->tables, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice), "hipMemcpy(tables)");
   if (ok) chk(hipMalloc(&c->clk, sizeof(Clock)), "hipMalloc(clock)");
-  if (ok) chk(hipMalloc(&c->cnt, sizeof(Counters)), "hipMalloc(counters)");
+  if (ok) chk(hipMalloc(&c->cnt, sizeof(Counters) * ATOMIC_SLOTS), "hipMalloc(counters)");
   if (ok) chk(hipHostMalloc(&c->hclk, sizeof(Clock), hipHostMallocDefault), "hipHostMalloc(clock)");
-  if (ok) chk(hipHostMalloc(&c->hcnt, sizeof(Counters), hipHostMallocDefault), "hipHostMalloc(counters)");
+  if (ok) chk(hipHostMalloc(&c->hcnt, sizeof(Counters) * ATOMIC_SLOTS, hipHostMallocDefault), "hipHostMalloc(counters)");
   if (!ok) {
     ecg_destroy(c);
     return nullptr;
@@ -378,7 +391,7 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   }
   // Zero the state and counters, reset the clock.
   for (int b = 0; b < nbuf; b++) hipMemset(c->W[b], 0, bytes);
-  hipMemset(c->cnt, 0, sizeof(Counters));
+  hipMemset(c->cnt, 0, sizeof(Counters) * ATOMIC_SLOTS);
   if (ecg_reset_clock(c) != ECG_OK) {
     ecg_destroy(c);
     return nullptr;
@@ -417,14 +430,12 @@ static Clock fresh_clock(double dt, int64_t max_iter, double t_end) {
   k.t_end = t_end;
   k.iteration = 0;
   k.max_iter = max_iter;
-  double big = 1.e10;
-  std::memcpy(&k.dtmax_bits, &big, sizeof(big));
   k.done = 0;
   return k;
 }
 
 static int set_clock(ecg_ctx* c, const Clock& k) {
-  hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(64), 0, c->stream, c->clk, k);
+  hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(64), 0, c->stream, c->clk, c->cnt, k);
   HIPCHK(hipGetLastError());
   return ECG_OK;
 }
@@ -524,7 +535,7 @@ static int enqueue_one(ecg_ctx* c) {
     L.wn = c->W[c->cur]; L.wsrc = c->W[2]; L.wout = c->W[nxt]; L.stage = 2;
     timed(c, ECG_K_CORRECT, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream); });
   }
-  timed(c, ECG_K_CLOCK, [&] { launch_clock(c->clk, c->cfg.cfl, c->stream); });
+  timed(c, ECG_K_CLOCK, [&] { launch_clock(c->clk, c->cnt, c->cfg.cfl, c->stream); });
   HIPCHK(hipGetLastError());
   c->cur = nxt;
   c->pending += 1;
@@ -533,7 +544,7 @@ static int enqueue_one(ecg_ctx* c) {
 
 static int read_clock(ecg_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->hclk, c->clk, sizeof(Clock), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hcnt, c->cnt, sizeof(Counters) * ATOMIC_SLOTS, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   resolve_events(c);
   return ECG_OK;
@@ -551,18 +562,35 @@ static int settle(ecg_ctx* c) {
   return ECG_OK;
 }
 
+// Host sum of the counter slots (fixed order).
+static Counters sum_counters(const ecg_ctx* c) {
+  Counters s;
+  std::memset(&s, 0, sizeof(s));
+  for (int q = 0; q < ATOMIC_SLOTS; q++) {
+    const Counters& h = c->hcnt[q];
+    s.newton_fail += h.newton_fail;
+    s.newton_iters += h.newton_iters;
+    s.relax_calls += h.relax_calls;
+    s.warm_calls += h.warm_calls;
+    s.warm_fallbacks += h.warm_fallbacks;
+    s.resid_sum += h.resid_sum;
+  }
+  return s;
+}
+
 static void fill_stats(ecg_ctx* c, ecg_step_stats* st) {
   if (!st) return;
   st->t = c->hclk->t;
   st->dt_next = c->hclk->dt_next;
   st->dt_max = c->hclk->dt_max;
   st->iteration = c->hclk->iteration;
-  st->newton_fail = (int64_t)c->hcnt->newton_fail;
-  st->newton_iters = (int64_t)c->hcnt->newton_iters;
-  st->relax_calls = (int64_t)c->hcnt->relax_calls;
-  st->warm_calls = (int64_t)c->hcnt->warm_calls;
-  st->warm_fallbacks = (int64_t)c->hcnt->warm_fallbacks;
-  st->resid_sum = c->hcnt->resid_sum;
+  const Counters s = sum_counters(c);
+  st->newton_fail = (int64_t)s.newton_fail;
+  st->newton_iters = (int64_t)s.newton_iters;
+  st->relax_calls = (int64_t)s.relax_calls;
+  st->warm_calls = (int64_t)s.warm_calls;
+  st->warm_fallbacks = (int64_t)s.warm_fallbacks;
+  st->resid_sum = s.resid_sum;
 }
 
 int ecg_step(ecg_ctx* c, double dt, double* dtmax_out, ecg_step_stats* st) {
@@ -571,13 +599,11 @@ int ecg_step(ecg_ctx* c, double dt, double* dtmax_out, ecg_step_stats* st) {
   int rc = read_clock(c);
   if (rc != ECG_OK) return rc;
   Clock k = *c->hclk;
-  const int64_t fails_before = (int64_t)c->hcnt->newton_fail;
+  const int64_t fails_before = (int64_t)sum_counters(c).newton_fail;
   k.dt = dt;
   k.done = 0;
   k.max_iter = 0;
   k.t_end = INFINITY;
-  double big = 1.e10;
-  std::memcpy(&k.dtmax_bits, &big, sizeof(big));
   rc = set_clock(c, k);
   if (rc != ECG_OK) return rc;
   rc = enqueue_one(c);
@@ -587,7 +613,7 @@ int ecg_step(ecg_ctx* c, double dt, double* dtmax_out, ecg_step_stats* st) {
   c->pending = 0;
   if (dtmax_out) *dtmax_out = c->hclk->dt_max;
   fill_stats(c, st);
-  const int64_t nfail = (int64_t)c->hcnt->newton_fail - fails_before;
+  const int64_t nfail = (int64_t)sum_counters(c).newton_fail - fails_before;
   return nfail > 0 ? (int)nfail : ECG_OK;
 }
 

@@ -54,6 +54,6 @@ typedef void (*fill_fn)(const Params& P, const DomainDev* doms, int ndom, void* 
 // Tile extents (cells per block along x, y, z) for a problem dimension.
 void tile_extent(int dim, int t[3]);
 
-void launch_clock(Clock* clk, double cfl, hipStream_t s);
+void launch_clock(Clock* clk, Counters* cnt, double cfl, hipStream_t s);
 
 }  // namespace ecg

@@ -66,17 +66,24 @@ struct Clock {
   double t, dt, dt_next, dt_max;
   double t_end;
   int64_t iteration, max_iter;
-  uint64_t dtmax_bits;      // atomicMin target (non-negative doubles order as integers)
   int done;
   int pad;
 };
 
+// Per-step reductions of the step kernels.  Blocks spread their atomics over
+// ATOMIC_SLOTS slots, one 128-byte line each (block b uses slot b % ATOMIC_SLOTS),
+// so that half a million blocks per launch do not serialise on one line; the
+// clock kernel reduces the dtMax slots, the host sums the counters.
+constexpr int ATOMIC_SLOTS = 64;
 struct Counters {
   unsigned long long newton_fail, newton_iters, relax_calls;
   unsigned long long warm_calls;      // relaxations started from the learned guess
   unsigned long long warm_fallbacks;  // of those, cold restarts after WARM_MAX_IT iterations
   double resid_sum;                   // sum over relaxed cells of |f| at exit
+  uint64_t dtmax_bits;                // min of the step's dtMax (non-negative doubles order as integers)
+  uint64_t pad[9];
 };
+static_assert(sizeof(Counters) == 128, "one counter slot per 128-byte line");
 
 // std::min / std::max tie semantics.
 template <typename R> __device__ __forceinline__ R dmin(R a, R b) { return (b < a) ? b : a; }
@@ -250,6 +257,16 @@ __device__ __forceinline__ void derive(const Params& P, Cell<N, R>& s, bool rebu
     cf += Y * s.c[k] * s.c[k];
   }
   s.rho = rho; s.pm = pm; s.em = em; s.cF = fsqrt(cf);
+}
+
+// Mixture density alone (the part of the derived state the cell update and the
+// energy correction read), summed in derive's order.
+template <int N>
+__device__ __forceinline__ void mixture_density(Cell<N>& s) {
+  double rho = 0.;
+#pragma unroll
+  for (int k = 0; k < N; k++) rho += s.a[k] * s.r[k];
+  s.rho = rho;
 }
 
 // verifyAndCorrectPhase (PhaseUEq.cpp:355-370, PhaseEuler.cpp:395-399)
@@ -556,7 +573,6 @@ __device__ __forceinline__ void build_prim(const Params& P, Cons<N>& A, Cell<N>&
     const double e = s.E - 0.5 * sqn3<double>(s.u);
     s.e[0] = e;
     s.p[0] = eos_pressure<double>(P.eos[0], m, e);
-    s.c[0] = eos_sound<double>(P.eos[0], P.eps, m, s.p[0]);
     return;
   }
   double un = 0., rhoMel = 0.;
@@ -590,7 +606,9 @@ __device__ __forceinline__ void build_prim(const Params& P, Cons<N>& A, Cell<N>&
   if (fabs(s.u[0]) < 1.e-8) s.u[0] = 0.;
   if (fabs(s.u[1]) < 1.e-8) s.u[1] = 0.;
   if (fabs(s.u[2]) < 1.e-8) s.u[2] = 0.;
-  derive<MODEL, N, double>(P, s, false);
+  // Of the derived state only the mixture density is read afterwards (energy
+  // correction); the rest is rebuilt from the stored primitives when needed.
+  mixture_density<N>(s);
   s.E = A.energM / rhoMel;
 }
 
@@ -653,7 +671,7 @@ __device__ __forceinline__ int relax_newton(const Params& P, Cell<N>& s, double 
     // Interface-pressure fallback (RelaxationPInfinite.cpp:82-97, RelaxationP.cpp:72-93)
     double z[N], pI = 0., sumZk = 0.;
 #pragma unroll
-    for (int k = 0; k < N; k++) z[k] = rk[k] * s.c[k];
+    for (int k = 0; k < N; k++) z[k] = rk[k] * eos_sound<double>(P.eos[k], P.eps, rk[k], pk[k]);
 #pragma unroll
     for (int k = 0; k < N; k++) {
       sumZk += z[k];
@@ -707,7 +725,10 @@ __device__ __forceinline__ double relax_guess(const Cell<N>& s) {
   return p;
 }
 
-// Cell::correctionEnergy + FluxPUEq::correctionEnergy (Kapila only), then fulfillState.  FP64.
+// Cell::correctionEnergy + FluxPUEq::correctionEnergy (Kapila only), then the
+// part of fulfillState that changes stored variables: Euler rebuilds E from (rho,
+// p) (PhaseEuler::extendedCalculusPhase).  The multiphase derived quantities are
+// not stored and are rebuilt from the primitives by their next reader.  FP64.
 template <int MODEL, int N>
 __device__ __forceinline__ void correct_and_derive(const Params& P, Cell<N>& s) {
   if (MODEL == M_PUEQ) {
@@ -725,7 +746,7 @@ __device__ __forceinline__ void correct_and_derive(const Params& P, Cell<N>& s) 
       verify_correct<MODEL, N, double>(P, s, k);
     }
   }
-  derive<MODEL, N, double>(P, s, MODEL == M_EULER);
+  if (MODEL == M_EULER) derive<MODEL, N, double>(P, s, true);
 }
 
 }  // namespace ecg

@@ -416,20 +416,24 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
 #pragma unroll
   for (int q = 0; q < MLP_IN; q++) feat[q] = 0.f;
   if (own_ok) {
-    // Boundary faces see the stage's own cell: W^n (first order / predictor) or W* (corrector).
-    Cell<N, R> src;
-    load_prim<MODEL, N, D>(P, (O2 && L.stage == 2) ? L.wsrc : L.wn, c, src);
-    derive<MODEL, N, R>(P, src, false);
     const int i = own[0], j = own[1], k = own[2];
-    if (i == 0) boundary_face<MODEL, N, D, FR_XM, R>(P, src, P.bc[0], R(P.coef[0]), an, pn, dtm, acc);
-    if (i == P.nx - 1) boundary_face<MODEL, N, D, FR_XP, R>(P, src, P.bc[1], R(P.coef[0]), an, pn, dtm, acc);
-    if (D >= 2) {
-      if (j == 0) boundary_face<MODEL, N, D, FR_YM, R>(P, src, P.bc[2], R(P.coef[1]), an, pn, dtm, acc);
-      if (j == P.ny - 1) boundary_face<MODEL, N, D, FR_YP, R>(P, src, P.bc[3], R(P.coef[1]), an, pn, dtm, acc);
-    }
-    if (D >= 3) {
-      if (k == 0) boundary_face<MODEL, N, D, FR_ZM, R>(P, src, P.bc[4], R(P.coef[2]), an, pn, dtm, acc);
-      if (k == P.nz - 1) boundary_face<MODEL, N, D, FR_ZP, R>(P, src, P.bc[5], R(P.coef[2]), an, pn, dtm, acc);
+    const bool on_bnd = (i == 0) || (i == P.nx - 1) || (D >= 2 && (j == 0 || j == P.ny - 1)) ||
+                        (D >= 3 && (k == 0 || k == P.nz - 1));
+    if (on_bnd) {
+      // Boundary faces see the stage's own cell: W^n (first order / predictor) or W* (corrector).
+      Cell<N, R> src;
+      load_prim<MODEL, N, D>(P, (O2 && L.stage == 2) ? L.wsrc : L.wn, c, src);
+      derive<MODEL, N, R>(P, src, false);
+      if (i == 0) boundary_face<MODEL, N, D, FR_XM, R>(P, src, P.bc[0], R(P.coef[0]), an, pn, dtm, acc);
+      if (i == P.nx - 1) boundary_face<MODEL, N, D, FR_XP, R>(P, src, P.bc[1], R(P.coef[0]), an, pn, dtm, acc);
+      if (D >= 2) {
+        if (j == 0) boundary_face<MODEL, N, D, FR_YM, R>(P, src, P.bc[2], R(P.coef[1]), an, pn, dtm, acc);
+        if (j == P.ny - 1) boundary_face<MODEL, N, D, FR_YP, R>(P, src, P.bc[3], R(P.coef[1]), an, pn, dtm, acc);
+      }
+      if (D >= 3) {
+        if (k == 0) boundary_face<MODEL, N, D, FR_ZM, R>(P, src, P.bc[4], R(P.coef[2]), an, pn, dtm, acc);
+        if (k == P.nz - 1) boundary_face<MODEL, N, D, FR_ZP, R>(P, src, P.bc[5], R(P.coef[2]), an, pn, dtm, acc);
+      }
     }
 
     // FP64 cell update from the time-n primitives.
@@ -448,7 +452,9 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
 #pragma unroll
       for (int k = 0; k < N; k++) own64.a[k] /= un;
     }
-    derive<MODEL, N, double>(P, own64, false);
+    // U^n needs the mixture density of the time-n cell (Cell::timeEvolution rebuilds
+    // the conservative variables from the primitives, FluxUEq::buildCons).
+    if (MODEL != M_EULER) mixture_density<N>(own64);
     const double scale = (O2 && L.stage == 1) ? 0.5 * dt : dt;
     Cons<N> A;
     time_evolution<MODEL, N, D, R>(P, acc, A, own64, scale);
@@ -517,16 +523,17 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
       m = dmin<double>(m, s_dt[q]); rs += s_res[q];
       a += s_c[0][q]; b += s_c[1][q]; r += s_c[2][q]; w += s_c[3][q]; wf += s_c[4][q];
     }
-    atomicMin(&clk->dtmax_bits, (unsigned long long)__double_as_longlong(m));
+    Counters* slot = cnt + (blockIdx.x % ATOMIC_SLOTS);
+    atomicMin(&slot->dtmax_bits, (unsigned long long)__double_as_longlong(m));
     if (r) {
-      atomicAdd(&cnt->newton_iters, a);
-      atomicAdd(&cnt->newton_fail, b);
-      atomicAdd(&cnt->relax_calls, r);
-      atomicAdd(&cnt->resid_sum, rs);
+      atomicAdd(&slot->newton_iters, a);
+      atomicAdd(&slot->newton_fail, b);
+      atomicAdd(&slot->relax_calls, r);
+      atomicAdd(&slot->resid_sum, rs);
     }
     if (w) {
-      atomicAdd(&cnt->warm_calls, w);
-      atomicAdd(&cnt->warm_fallbacks, wf);
+      atomicAdd(&slot->warm_calls, w);
+      atomicAdd(&slot->warm_fallbacks, wf);
     }
   }
 }
