@@ -40,10 +40,11 @@ static fill_fn get_fill_fn(int model, int nph, int dim, bool f32) {
   return nullptr;
 }
 
-// Cells per 256-thread tile (must match Tile<D> in ecg_step_impl.hpp).
-void tile_extent(int dim, int t[3]) {
+// Cells per 256-thread tile (must match Tile<D, ES> in ecg_step_impl.hpp).
+void tile_extent(int dim, int esize, int t[3]) {
   if (dim == 1) { t[0] = 256; t[1] = 1; t[2] = 1; }
   else if (dim == 2) { t[0] = 16; t[1] = 16; t[2] = 1; }
+  else if (esize == 8) { t[0] = 8; t[1] = 8; t[2] = 4; }
   else { t[0] = 16; t[1] = 4; t[2] = 4; }
 }
 
@@ -343,7 +344,7 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   for (int i = 0; i < 4; i++) P.lim[i] = g.lim[i];
   for (int k = 0; k < g.nph; k++) P.eos[k] = make_eos(g.eos[k]);
   int tex[3];
-  tile_extent(c->D, tex);
+  tile_extent(c->D, c->esize, tex);
   for (int a = 0; a < 3; a++) P.tiles[a] = (n3[a] + tex[a] - 1) / tex[a];
 
   // Distance tables of the interior faces (Element.cpp:77-81, Face.cpp:52-56).

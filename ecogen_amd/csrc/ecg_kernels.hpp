@@ -51,8 +51,8 @@ struct StepLaunchT {
 typedef void (*step_fn)(const Params& P, const StepLaunch& L, Clock* clk, Counters* cnt, hipStream_t s);
 typedef void (*fill_fn)(const Params& P, const DomainDev* doms, int ndom, void* w, hipStream_t s);
 
-// Tile extents (cells per block along x, y, z) for a problem dimension.
-void tile_extent(int dim, int t[3]);
+// Tile extents (cells per block along x, y, z) for a problem dimension and state element size.
+void tile_extent(int dim, int esize, int t[3]);
 
 void launch_clock(Clock* clk, Counters* cnt, double cfl, hipStream_t s);
 

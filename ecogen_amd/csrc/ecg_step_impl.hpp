@@ -24,13 +24,18 @@
 
 namespace ecg {
 
-template <int D> struct Tile;
-template <> struct Tile<1> { static constexpr int X = 256, Y = 1, Z = 1; };
-template <> struct Tile<2> { static constexpr int X = 16, Y = 16, Z = 1; };
-template <> struct Tile<3> { static constexpr int X = 16, Y = 4, Z = 4; };
+// Cells per 256-thread tile by dimension and state element size.  3D FP64 tiles
+// are 8 x 8 x 4 (fewest halo faces per cell), 3D FP32 tiles 16 x 4 x 4 (longer
+// contiguous rows for the narrower rows of floats); both measured faster for
+// their form on the 512^3 case.
+template <int D, int ES> struct Tile;
+template <int ES> struct Tile<1, ES> { static constexpr int X = 256, Y = 1, Z = 1; };
+template <int ES> struct Tile<2, ES> { static constexpr int X = 16, Y = 16, Z = 1; };
+template <> struct Tile<3, 8> { static constexpr int X = 8, Y = 8, Z = 4; };
+template <> struct Tile<3, 4> { static constexpr int X = 16, Y = 4, Z = 4; };
 
-template <int D, int A> struct TileAxis {
-  static constexpr int T = (A == 0) ? Tile<D>::X : (A == 1) ? Tile<D>::Y : Tile<D>::Z;
+template <int D, int A, int ES> struct TileAxis {
+  static constexpr int T = (A == 0) ? Tile<D, ES>::X : (A == 1) ? Tile<D, ES>::Y : Tile<D, ES>::Z;
   static constexpr int LINES = 256 / T;
   static constexpr int NSLOPE = (T + 3) * LINES;
   static constexpr int NFACE = (T + 1) * LINES;
@@ -46,23 +51,23 @@ template <int MODEL, int N, int D> struct Sizes {
   static constexpr int NS = PV + D;
   // flux: alpha, mass, energ per phase (Euler: mass), momentum (D), energy, sM, uStar
   static constexpr int NF = ((MODEL == M_EULER) ? 1 : 3 * N) + D + 1 + 2;
-  // Elements staged per line of axis A.
-  template <int A> static constexpr int per_line(bool o2) {
-    return (o2 ? (TileAxis<D, A>::T + 3) * NS : 0) + (TileAxis<D, A>::T + 1) * NF;
+  // Elements staged per line of axis A (tiles of element size ES).
+  template <int A, int ES> static constexpr int per_line(bool o2) {
+    return (o2 ? (TileAxis<D, A, ES>::T + 3) * NS : 0) + (TileAxis<D, A, ES>::T + 1) * NF;
   }
-  // Batches of lines of axis A for elements of esize bytes.
-  template <int A> static constexpr int batches(bool o2, int esize) {
+  // Batches of lines of axis A.
+  template <int A, int ES> static constexpr int batches(bool o2) {
     int nb = 1;
-    while (nb < TileAxis<D, A>::LINES && per_line<A>(o2) * (TileAxis<D, A>::LINES / nb) * esize > LDS_BATCH_BYTES)
+    while (nb < TileAxis<D, A, ES>::LINES && per_line<A, ES>(o2) * (TileAxis<D, A, ES>::LINES / nb) * ES > LDS_BATCH_BYTES)
       nb *= 2;
     return nb;
   }
-  template <int A> static constexpr int lds_axis(bool o2, int esize) {
-    return per_line<A>(o2) * (TileAxis<D, A>::LINES / batches<A>(o2, esize));
+  template <int A, int ES> static constexpr int lds_axis(bool o2) {
+    return per_line<A, ES>(o2) * (TileAxis<D, A, ES>::LINES / batches<A, ES>(o2));
   }
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
-  static constexpr int lds_elems(bool o2, int esize) {
-    return cmax(lds_axis<0>(o2, esize), cmax(D >= 2 ? lds_axis<1>(o2, esize) : 0, D >= 3 ? lds_axis<2>(o2, esize) : 0));
+  template <int ES> static constexpr int lds_elems(bool o2) {
+    return cmax(lds_axis<0, ES>(o2), cmax(D >= 2 ? lds_axis<1, ES>(o2) : 0, D >= 3 ? lds_axis<2, ES>(o2) : 0));
   }
 };
 
@@ -209,7 +214,7 @@ __device__ __forceinline__ void flux_from_lds(const R* o, Flux<N, R>& F) {
 
 // Tile origin of the block, XCD-aware: when the tile count divides by the 8
 // XCDs, each XCD receives a contiguous range of tiles (neighbours share an L2).
-template <int D>
+template <int D, int ES>
 __device__ __forceinline__ void tile_origin(const Params& P, int& i0, int& j0, int& k0) {
   const int nb = P.tiles[0] * P.tiles[1] * P.tiles[2];
   int b = blockIdx.x;
@@ -218,9 +223,9 @@ __device__ __forceinline__ void tile_origin(const Params& P, int& i0, int& j0, i
   const int rest = b / P.tiles[0];
   const int ty = rest % P.tiles[1];
   const int tz = rest / P.tiles[1];
-  i0 = tx * Tile<D>::X;
-  j0 = ty * Tile<D>::Y;
-  k0 = tz * Tile<D>::Z;
+  i0 = tx * Tile<D, ES>::X;
+  j0 = ty * Tile<D, ES>::Y;
+  k0 = tz * Tile<D, ES>::Z;
 }
 
 // One axis pass (steps A, B, C above) for axis AX with face frame FR, in
@@ -229,14 +234,15 @@ template <int MODEL, int N, int D, bool O2, typename R, int LK, int AX, int FR>
 __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>& L, R* lds, const int org[3],
                                           const int own[3], bool own_ok, const R* an, const R* pn, R& dtm,
                                           ConsAcc<N, R>& acc) {
-  using TA = TileAxis<D, AX>;
+  constexpr int ES = (int)sizeof(R);
+  using TA = TileAxis<D, AX, ES>;
   using SZ = Sizes<MODEL, N, D>;
   constexpr int T = TA::T;
-  constexpr int NB = SZ::template batches<AX>(O2, (int)sizeof(R));
+  constexpr int NB = SZ::template batches<AX, ES>(O2);
   constexpr int LB = TA::LINES / NB;              // lines per batch
   constexpr int B = (AX == 0) ? 1 : 0;           // first line axis
   constexpr int C = (AX == 2) ? 1 : 2;           // second line axis
-  constexpr int TB = (B == 0) ? Tile<D>::X : Tile<D>::Y;
+  constexpr int TB = (B == 0) ? Tile<D, ES>::X : Tile<D, ES>::Y;
   const int n3[3] = {P.nx, P.ny, P.nz};
   const int64_t st3[3] = {1, (int64_t)P.nx, (int64_t)P.nx * P.ny};
   const int na = n3[AX];
@@ -382,12 +388,13 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
   if (clk->done) return;
   const double dt = clk->dt;
   int org[3];
-  tile_origin<D>(P, org[0], org[1], org[2]);
+  constexpr int ES = (int)sizeof(R);
+  tile_origin<D, ES>(P, org[0], org[1], org[2]);
   const int tid = threadIdx.x;
   int own[3];
-  own[0] = org[0] + tid % Tile<D>::X;
-  own[1] = org[1] + (tid / Tile<D>::X) % Tile<D>::Y;
-  own[2] = org[2] + tid / (Tile<D>::X * Tile<D>::Y);
+  own[0] = org[0] + tid % Tile<D, ES>::X;
+  own[1] = org[1] + (tid / Tile<D, ES>::X) % Tile<D, ES>::Y;
+  own[2] = org[2] + tid / (Tile<D, ES>::X * Tile<D, ES>::Y);
   const bool own_ok = own[0] < P.nx && own[1] < P.ny && own[2] < P.nz;
   const int64_t c = own[0] + (int64_t)own[1] * P.nx + (int64_t)own[2] * P.nx * P.ny;
 
@@ -604,10 +611,11 @@ template <int MODEL, int N, int D, bool O2, typename R, bool LEARN, int LK>
 void launch_step_v(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counters* cnt, hipStream_t s) {
   using SZ = Sizes<MODEL, N, D>;
   const int blocks = P.tiles[0] * P.tiles[1] * P.tiles[2];
-  size_t lds = sizeof(R) * SZ::lds_elems(O2, (int)sizeof(R));
+  size_t lds = sizeof(R) * SZ::template lds_elems<(int)sizeof(R)>(O2);
   if (LEARN && lds < (size_t)MLP_BLOCK_LDS) lds = MLP_BLOCK_LDS;
   auto k = k_step<MODEL, N, D, O2, R, LEARN, LK>;
-  hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  // A failure here surfaces as the launch error the caller checks next.
+  (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
 }
 
