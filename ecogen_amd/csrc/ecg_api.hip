@@ -726,6 +726,7 @@ int ecg_attach_samples(ecg_ctx* c, ecg_samples* smp, int stride) {
   }
   if (stride < 1) return fail(ECG_E_ARG, "sample stride must be at least 1");
   if (c->cfg.model == ECG_EULER || !c->cfg.relax_p) return fail(ECG_E_ARG, "the case has no pressure relaxation");
+  if (c->mlp.frag) return fail(ECG_E_ARG, "samples record cold solves: detach the warm start first");
   // Records written so far stay; the candidate counter restarts for this attachment.
   const long long held = samples_count(smp);
   if (held < 0) return fail(ECG_E_HIP, "reading the sample counter failed");
@@ -744,6 +745,7 @@ int ecg_set_warm_start(ecg_ctx* c, const ecg_mlp* m) {
     return ECG_OK;
   }
   if (c->cfg.precision != ECG_FP32_LEARNED) return fail(ECG_E_ARG, "the warm start belongs to the ECG_FP32_LEARNED form");
+  if (c->sink.buf) return fail(ECG_E_ARG, "samples record cold solves: detach the sample buffer first");
   HIPCHK(hipStreamSynchronize(m->stream));
   c->mlp = MlpDev{m->frag, m->bias};
   return ECG_OK;

@@ -173,28 +173,30 @@ __device__ __forceinline__ void mlp_infer_tile(const MlpDev& M, const unsigned s
   const v4i16 bx = as_v4i16(xin);
   const float* b1 = M.bias;
   const float* b2 = M.bias + MLP_H;
-  v4i16 h1[4], h2[4];
+  v4i16 h1[4];
 #pragma unroll
   for (int mt = 0; mt < 4; mt++) {
     v4f32 acc = {0.f, 0.f, 0.f, 0.f};
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(as_v4i16(M.frag[(MLP_FRAG_W1 + mt) * 64 + lane]), bx, acc, 0, 0, 0);
     h1[mt] = act_frag(acc, b1, mt, lane);
   }
-#pragma unroll
+  // Layer 3 consumes each 16-row block of layer 2 as soon as it is activated.
+  // The loop stays rolled: unrolled, the compiler interleaves the 16 GELUs of
+  // all four blocks and the inference alone needs 88 VGPRs instead of 48,
+  // which costs the fused step kernel a wave per SIMD.
+  v4f32 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
   for (int mt = 0; mt < 4; mt++) {
-    v4f32 acc = {0.f, 0.f, 0.f, 0.f};
+    asm volatile("" ::: "memory");
+    v4f32 a2 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kt = 0; kt < 4; kt++) {
       const v4i16 a = as_v4i16(M.frag[(MLP_FRAG_W2 + mt * 4 + kt) * 64 + lane]);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[kt], acc, 0, 0, 0);
+      a2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[kt], a2, 0, 0, 0);
     }
-    h2[mt] = act_frag(acc, b2, mt, lane);
+    const v4i16 h2 = act_frag(a2, b2, mt, lane);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(as_v4i16(M.frag[(MLP_FRAG_W3 + mt) * 64 + lane]), h2, acc, 0, 0, 0);
   }
-  v4f32 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int kt = 0; kt < 4; kt++)
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(as_v4i16(M.frag[(MLP_FRAG_W3 + kt) * 64 + lane]), h2[kt], acc, 0, 0,
-                                                    0);
   // Row 0 of the output tile: lanes 0..15, element 0.
   if (lane < 16) ystage[nt * 16 + lane] = acc[0] + M.bias[2 * MLP_H];
 }

@@ -498,7 +498,9 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
       n_relax = 1; n_it = (unsigned long long)it; n_fail = (it >= 100) ? 1 : 0; resid = fabs(fres);
     }
   }
-  if (MODEL != M_EULER && L.sink.buf != nullptr) {
+  // Capture records cold solves only (the API keeps a warm start and a sink apart),
+  // so the features need not stay live across the inference.
+  if (!LEARN && MODEL != M_EULER && L.sink.buf != nullptr) {
     const bool cap = spread_ok && it > 0 && it < 100;
     const double t = cap ? (nw.p[0] - p_cold) / spread : 0.;
     const float rem = cap ? (float)((t - t_lin_d) / sigma) : 0.f;

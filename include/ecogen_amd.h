@@ -220,7 +220,8 @@ int ecg_samples_upload(ecg_samples* s, const float* host, int64_t n);
  * cell that relaxes (cold start, any precision form) from distinct phase
  * pressures, after the records the buffer already holds (attaching restarts
  * the candidate count; records beyond the capacity are dropped).  One buffer
- * serves one attached context at a time.  NULL detaches. */
+ * serves one attached context at a time.  NULL detaches.  Records come from cold
+ * solves: a context with a warm start set (ecg_set_warm_start) refuses a buffer. */
 int ecg_attach_samples(ecg_ctx* ctx, ecg_samples* s, int stride);
 
 /* 12 -> 64 -> 64 -> 1 GELU network predicting u = (t - t_lin) / sigma (clipped to +-4 in training).
@@ -240,7 +241,8 @@ int ecg_mlp_train(ecg_mlp* m, ecg_samples* s, int epochs, int batch, float lr, u
 int ecg_mlp_predict(ecg_mlp* m, ecg_samples* s, int64_t n, float* out);
 /* Use m's guess, p = p_cold + s (t_lin + sigma y), in the relaxations of an
  * ECG_FP32_LEARNED context; Newton still runs to |f| <= 1e-10 and restarts from
- * the cold guess after 30 evaluations.  NULL returns to the cold start. */
+ * the cold guess after 30 evaluations.  NULL returns to the cold start.  Refused
+ * while a sample buffer is attached. */
 int ecg_set_warm_start(ecg_ctx* ctx, const ecg_mlp* m);
 
 #ifdef __cplusplus

@@ -116,6 +116,29 @@ def test_mfma_inference_matches_bf16_oracle():
 
 
 @pytest.mark.gpu
+def test_samples_and_warm_start_are_exclusive():
+    """Sample records describe cold solves: a context takes a sample buffer or a
+    warm start, not both at once."""
+    from ecogen_amd import EcgError, Mlp, Samples, Solver
+    print("[learned] sample buffer / warm start exclusivity", flush=True)
+    c = dict(kapila_bubbles_2d(n=32, nbub=2, order=1))
+    c["precision"] = "fp32_learned"
+    s = Solver(c)
+    smp = Samples(1000)
+    m = Mlp(seed=2)
+    s.set_warm_start(m)
+    with pytest.raises(EcgError, match="detach the warm start"):
+        s.attach_samples(smp, 1)
+    s.set_warm_start(None)
+    s.attach_samples(smp, 1)
+    with pytest.raises(EcgError, match="detach the sample buffer"):
+        s.set_warm_start(m)
+    s.attach_samples(None)
+    s.set_warm_start(m)
+    s.run(max_iter=2)
+
+
+@pytest.mark.gpu
 def test_trained_warm_start_saves_newton_iterations():
     """Train on samples of the plain FP64 solver, then compare the FP32 form's
     Newton work with the cold guess, the linearised guess (network output zero)
