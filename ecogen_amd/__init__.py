@@ -77,6 +77,14 @@ SIGNATURES = {
     "ecg_download_state": (C.c_int, [C.c_void_p, C.POINTER(_DP), C.c_int]),
     "ecg_step": (C.c_int, [C.c_void_p, C.c_double, _DP, C.POINTER(EcgStepStats)]),
     "ecg_run": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(EcgStepStats)]),
+    "ecg_run_to": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(EcgStepStats)]),
+    "ecg_set_time": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int64]),
+    "ecg_output_open": (C.c_void_p, [C.c_char_p, C.c_char_p, C.c_char_p]),
+    "ecg_output_write": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_double,
+                                   C.c_double]),
+    "ecg_output_restart": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "ecg_output_close": (None, [C.c_void_p]),
     "ecg_enqueue_steps": (C.c_int, [C.c_void_p, C.c_int]),
     "ecg_synchronize": (C.c_int, [C.c_void_p, C.POINTER(EcgStepStats)]),
     "ecg_reset_clock": (C.c_int, [C.c_void_p]),
@@ -269,6 +277,15 @@ class Solver:
         _check(self.L.ecg_run(self.h, int(max_iter), float(t_end), C.byref(self.stats)))
         return self.stats.as_dict()
 
+    def run_to(self, until_iter=0, until_t=0.0):
+        """Continue the device clock until iteration until_iter (> 0) or time until_t."""
+        _check(self.L.ecg_run_to(self.h, int(until_iter), float(until_t), C.byref(self.stats)))
+        return self.stats.as_dict()
+
+    def set_time(self, t, dt, iteration):
+        """Clock of a restart: time, dt of the next step, iteration count."""
+        _check(self.L.ecg_set_time(self.h, float(t), float(dt), int(iteration)))
+
     def reset_clock(self):
         _check(self.L.ecg_reset_clock(self.h))
 
@@ -327,6 +344,42 @@ class Solver:
 
 def _fp(a):
     return a.ctypes.data_as(_FP)
+
+
+class Output:
+    """The reference's result files for a case (ecg_output_*): host-side, no GPU needed."""
+
+    def __init__(self, case_dir, work_dir, eos_dir=None):
+        self.L = library()
+        self.h = self.L.ecg_output_open(str(case_dir).encode(), eos_dir.encode() if eos_dir else None,
+                                        str(work_dir).encode())
+        if not self.h:
+            raise EcgError(self.L.ecg_last_error().decode())
+
+    def write(self, state, iteration, t, dt_next, seconds=0.0):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        ptrs = (_DP * st.shape[0])(*[st[v].ctypes.data_as(_DP) for v in range(st.shape[0])])
+        _check(self.L.ecg_output_write(self.h, ptrs, st.shape[0], int(iteration), float(t), float(dt_next),
+                                       float(seconds)))
+
+    def restart(self, nvar, ncell):
+        """(state, iteration, t, dt) of the case's restart file."""
+        out = np.empty((nvar, ncell), dtype=np.float64)
+        ptrs = (_DP * nvar)(*[out[v].ctypes.data_as(_DP) for v in range(nvar)])
+        it, t, dt = C.c_int64(), C.c_double(), C.c_double()
+        _check(self.L.ecg_output_restart(self.h, ptrs, nvar, C.byref(it), C.byref(t), C.byref(dt)))
+        return out, it.value, t.value, dt.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ecg_output_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Samples:

@@ -29,7 +29,7 @@ COMMON = [
 
 
 HOST = os.path.join(HERE, "host")
-HOST_SOURCES = ["xml.cpp", "ecogen.cpp"]
+HOST_SOURCES = ["xml.cpp", "ecogen.cpp", "output.cpp"]
 RUNNER = os.path.join(HERE, "ecogen_run")
 
 

@@ -618,11 +618,12 @@ int ecg_step(ecg_ctx* c, double dt, double* dtmax_out, ecg_step_stats* st) {
   return nfail > 0 ? (int)nfail : ECG_OK;
 }
 
-int ecg_run(ecg_ctx* c, int64_t max_iter, double t_end, ecg_step_stats* st) {
-  if (!c) return fail(ECG_E_ARG, "null context");
-  if (max_iter <= 0 && !(t_end > 0)) return fail(ECG_E_ARG, "need max_iter > 0 or t_end > 0");
+// Runs clock k until the device marks it done (or max_steps steps, when > 0,
+// have been enqueued).  The executed steps decide which buffer holds the state.
+static int run_clock(ecg_ctx* c, const Clock& k, int64_t max_steps, ecg_step_stats* st) {
   const int start = c->cur;
-  int rc = set_clock(c, fresh_clock(1.e-15, max_iter, max_iter > 0 ? INFINITY : t_end));
+  const int64_t it0 = k.iteration;
+  int rc = set_clock(c, k);
   if (rc != ECG_OK) return rc;
   int64_t enq = 0;
   int batch = 8;
@@ -631,19 +632,56 @@ int ecg_run(ecg_ctx* c, int64_t max_iter, double t_end, ecg_step_stats* st) {
       rc = enqueue_one(c);
       if (rc != ECG_OK) return rc;
       enq++;
-      if (max_iter > 0 && enq >= max_iter) break;
+      if (max_steps > 0 && enq >= max_steps) break;
     }
     rc = read_clock(c);
     if (rc != ECG_OK) return rc;
     if (c->hclk->done) break;
     if (batch < 256) batch *= 2;
   }
-  // The executed steps decide which buffer holds the final state.
-  const int64_t done_steps = c->hclk->iteration;
+  const int64_t done_steps = c->hclk->iteration - it0;
   c->cur = (int)((start + done_steps) % 2);
   c->pending = 0;
   fill_stats(c, st);
   return ECG_OK;
+}
+
+int ecg_run(ecg_ctx* c, int64_t max_iter, double t_end, ecg_step_stats* st) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  if (max_iter <= 0 && !(t_end > 0)) return fail(ECG_E_ARG, "need max_iter > 0 or t_end > 0");
+  int rc = settle(c);
+  if (rc != ECG_OK) return rc;
+  return run_clock(c, fresh_clock(1.e-15, max_iter, max_iter > 0 ? INFINITY : t_end), max_iter, st);
+}
+
+int ecg_run_to(ecg_ctx* c, int64_t until_iter, double until_t, ecg_step_stats* st) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  if (until_iter <= 0 && !(until_t > 0)) return fail(ECG_E_ARG, "need until_iter > 0 or until_t > 0");
+  int rc = read_clock(c);
+  if (rc != ECG_OK) return rc;
+  c->pending = 0;
+  Clock k = *c->hclk;
+  if (until_iter > 0 && k.iteration >= until_iter) return fail(ECG_E_ARG, "the clock is already at until_iter");
+  if (until_iter <= 0 && k.t >= until_t) return fail(ECG_E_ARG, "the clock is already at until_t");
+  k.max_iter = until_iter > 0 ? until_iter : 0;
+  k.t_end = until_iter > 0 ? INFINITY : until_t;
+  k.done = 0;
+  return run_clock(c, k, until_iter > 0 ? until_iter - k.iteration : 0, st);
+}
+
+int ecg_set_time(ecg_ctx* c, double t, double dt, int64_t iteration) {
+  if (!c) return fail(ECG_E_ARG, "null context");
+  if (!(dt > 0) || !std::isfinite(dt) || !std::isfinite(t) || iteration < 0)
+    return fail(ECG_E_ARG, "need a finite time, a positive dt and a non-negative iteration");
+  int rc = read_clock(c);
+  if (rc != ECG_OK) return rc;
+  c->pending = 0;
+  Clock k = fresh_clock(dt, 0, INFINITY);
+  k.t = t;
+  k.iteration = iteration;
+  rc = set_clock(c, k);
+  if (rc != ECG_OK) return rc;
+  return read_clock(c);
 }
 
 int ecg_enqueue_steps(ecg_ctx* c, int n) {

@@ -2,6 +2,7 @@
 #include "ecogen.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cctype>
 #include <cerrno>
 #include <cmath>
@@ -9,6 +10,7 @@
 #include <cstring>
 #include <sys/stat.h>
 
+#include "output.hpp"
 #include "xml.hpp"
 
 namespace ecg {
@@ -139,6 +141,28 @@ void read_main(const std::string& file, CaseDescription& c) {
   if (doc->name != "computationParam") bad(file, "root element must be <computationParam>");
   const XmlElement& run = need(doc->child("run"), "run", file);
   c.control.runName = run.trimmed_text();
+  if (c.control.runName.empty()) bad(file, "empty <run>");
+  // Output mode, cuts, probes and global quantities (Input.cpp:104-146).
+  const XmlElement& om = need(doc->child("outputMode"), "outputMode", file);
+  const std::string* fmt = om.attr("format");
+  if (!fmt || fmt->empty()) bad(file, "<outputMode> needs attribute format");
+  c.output.format = upper(*fmt);
+  if (c.output.format != "XML" && c.output.format != "GNU") bad(file, "<outputMode> format must be XML or GNU");
+  c.output.binary = boolean(om, "binary", file);
+  if (has(om, "precision")) c.output.precision = (int)integer(om, "precision", file);
+  if (has(om, "reducedOutput")) c.output.reducedOutput = boolean(om, "reducedOutput", file);
+  if (c.output.format == "GNU" && c.output.binary)
+    unsupported(file, "OutputGNU::createFilenameGNU : binary data not planned");
+  for (const char* el : {"cut1D", "cut2D", "probe", "recordPsat", "recordBoundary"})
+    if (doc->child(el)) unsupported(file, std::string("the <") + el + "> output is not built");
+  for (const XmlElement* gq : doc->children_named("globalQuantity")) {
+    const std::string* q = gq->attr("quantity");
+    if (!q || q->empty()) bad(file, "<globalQuantity> needs attribute quantity");
+    std::string l = *q;
+    for (auto& ch : l) ch = (char)std::tolower((unsigned char)ch);
+    if (l != "mass" && l != "totalenergy") bad(file, "<globalQuantity> quantity must be mass or totalEnergy");
+    c.output.globalQuantities.push_back(l);
+  }
   const XmlElement& tc = need(doc->child("timeControlMode"), "timeControlMode", file);
   c.control.iterationsMode = boolean(tc, "iterations", file);
   if (c.control.iterationsMode) {
@@ -170,10 +194,18 @@ void read_main(const std::string& file, CaseDescription& c) {
     c.limiters[2] = Limiter::fromName(gvf, true);
     c.limiters[3] = Limiter::fromName(ivf, true);
   }
+  if (c.control.iterationsMode && c.control.iterFreq <= 0) bad(file, "<iterations> iterFreq must be positive");
   if (const XmlElement* r = doc->child("restartSimulation")) {
-    if (has(*r, "restartFileNumber") && integer(*r, "restartFileNumber", file) != 0)
-      unsupported(file, "restarting from an output file is not built");
+    // Input.cpp:304-311; AMRsaveFreq is required by the reference even without AMR.
+    c.output.restartFileNumber = (int)integer(*r, "restartFileNumber", file);
+    (void)integer(*r, "AMRsaveFreq", file);
+    if (c.output.restartFileNumber > 0 && c.output.format != "XML")
+      unsupported(file, "Run::restartSimulation: Restart option only available for XML output");
+    if (c.output.restartFileNumber > 0 && c.output.binary)
+      unsupported(file, "resuming on binary results file not available");
   }
+  if (const XmlElement* m = doc->child("meshMappingRestart"))
+    if (has(*m, "restartFileNumber")) unsupported(file, "restart with mesh mapping is not built");
 }
 
 void read_mesh(const std::string& file, CaseDescription& c) {
@@ -403,8 +435,11 @@ void read_initial_conditions(const std::string& file, CaseDescription& c) {
 
 CaseDescription Input::readInputXML(const std::string& caseDir, const std::string& eosDir, int precision) {
   CaseDescription c;
+  c.caseDir = caseDir;
   read_main(join(caseDir, "main.xml"), c);
   read_mesh(join(caseDir, "mesh.xml"), c);
+  if (c.output.format == "GNU" && c.config.nz > 1)
+    unsupported(join(caseDir, "main.xml"), "OutputGNU::writeScriptGnuplot : print script gnuplot non prevu en 3D");
   read_model(join(caseDir, "model.xml"), eosDir, c);
   read_initial_conditions(join(caseDir, "initialConditions.xml"), c);
   ecg_config& g = c.config;
@@ -421,24 +456,95 @@ CaseDescription Input::readInputXML(const std::string& caseDir, const std::strin
   return c;
 }
 
-Run::Run(const CaseDescription& c) : m_case(c) {}
+Run::Run(const CaseDescription& c, const std::string& workFolder, bool writeOutput)
+    : m_case(c), m_work(workFolder), m_write(writeOutput) {}
 
 Run::~Run() {
   if (m_ctx) ecg_destroy(m_ctx);
 }
 
 void Run::initialize() {
+  // Run::initialize (Run.cpp:54-246): the output folders and the copies of the
+  // input files first, then the mesh and the initial conditions, then either
+  // the restart point or the first print (file 0).
+  const int restart = m_case.output.restartFileNumber;
+  if (m_write || restart > 0) {
+    m_out = Output::create(m_case, m_work);
+    m_out->copyInputFiles();
+  }
+  long long it = 0;
+  double t = 0., dt = 1.e-15;
+  if (restart > 0) m_out->readInfos(restart, it, t, dt);
   m_ctx = ecg_create(&m_case.config);
   if (!m_ctx) throw InputError(ECG_E_ARG, ecg_last_error());
-  const int rc = ecg_fill(m_ctx, m_case.domains.data(), (int)m_case.domains.size());
+  int rc = ecg_fill(m_ctx, m_case.domains.data(), (int)m_case.domains.size());
   if (rc != ECG_OK) throw InputError(rc, ecg_last_error());
+  if (m_out) m_out->initializeOutput();
+  if (restart > 0) {
+    // Run::restartSimulation (Run.cpp:345-381).
+    const State st = m_out->readResults();
+    std::vector<const double*> ptr;
+    for (const auto& v : st) ptr.push_back(v.data());
+    rc = ecg_upload_state(m_ctx, ptr.data(), (int)ptr.size());
+    if (rc != ECG_OK) throw InputError(rc, ecg_last_error());
+    rc = ecg_set_time(m_ctx, t, dt, it);
+    if (rc != ECG_OK) throw InputError(rc, ecg_last_error());
+    m_iteration = it;
+    m_time = t;
+    m_dtNext = dt;
+  } else if (m_write) {
+    m_out->initializeOutputInfos();
+    print(0.);
+    m_out->prepareErrorFiles();
+  }
+}
+
+void Run::print(double seconds) {
+  // The print block of Run::solver (Run.cpp:441-467): infoCalcul line, infoMesh
+  // line, global quantities, then the dataset.
+  const State st = state();
+  m_out->writeInfos(m_iteration, m_time, m_dtNext, seconds);
+  m_out->saveInfoCells((long long)m_case.config.nx * m_case.config.ny * m_case.config.nz);
+  m_out->writeGlobalQuantities(m_time, st);
+  m_out->writeResults(st);
 }
 
 ecg_step_stats Run::solver() {
+  // Run::solver (Run.cpp:385-498): the device runs the steps between two
+  // prints; print and stop rules are evaluated after each of them exactly as
+  // the reference does after every step.
   ecg_step_stats st{};
-  const RunControl& t = m_case.control;
-  const int rc = t.iterationsMode ? ecg_run(m_ctx, t.iterations, 0., &st) : ecg_run(m_ctx, 0, t.totalTime, &st);
-  if (rc < 0) throw InputError(rc, ecg_last_error());
+  const RunControl& tc = m_case.control;
+  const auto t0 = std::chrono::steady_clock::now();
+  double printNext = m_time + tc.timeFreq;
+  for (;;) {
+    int rc;
+    if (tc.iterationsMode) {
+      if (m_iteration >= tc.iterations) break;
+      long long target = tc.iterations;
+      if (m_write) target = std::min<long long>(target, (m_iteration / tc.iterFreq + 1) * (long long)tc.iterFreq);
+      rc = ecg_run_to(m_ctx, target, 0., &st);
+    } else {
+      if (m_time >= tc.totalTime && m_iteration > 0) break;
+      const double until = m_write ? std::min(printNext, tc.totalTime) : tc.totalTime;
+      rc = ecg_run_to(m_ctx, 0, until, &st);
+    }
+    if (rc < 0) throw InputError(rc, ecg_last_error());
+    m_iteration = st.iteration;
+    m_time = st.t;
+    m_dtNext = st.dt_next;
+    bool doPrint = false, done = false;
+    if (tc.iterationsMode) {
+      if (m_iteration % tc.iterFreq == 0) doPrint = true;
+      if (m_iteration >= tc.iterations) done = true;
+    } else {
+      if (m_time >= printNext) { doPrint = true; printNext += tc.timeFreq; }
+      if (m_time >= tc.totalTime) { doPrint = true; done = true; }
+    }
+    if (doPrint && m_write)
+      print(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    if (done) break;
+  }
   return st;
 }
 

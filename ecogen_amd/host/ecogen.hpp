@@ -70,9 +70,21 @@ struct RunControl {
   double cfl = 0.;
 };
 
+// <outputMode>, <globalQuantity> and <restartSimulation> of main.xml (Input.cpp:104-146, 304-311).
+struct OutputSettings {
+  std::string format = "XML";   // XML or GNU
+  bool binary = false;
+  int precision = 0;            // 0: the stream's default (6 significant digits)
+  bool reducedOutput = false;
+  std::vector<std::string> globalQuantities;   // "mass", "totalenergy", in document order
+  int restartFileNumber = 0;    // > 0: resume from that XML result file
+};
+
 // What the reference's Input::readInputXML assembles for the hot path.
 struct CaseDescription {
+  std::string caseDir;    // folder of the four case files
   RunControl control;
+  OutputSettings output;
   Model model;
   std::vector<Eos> eos;
   Limiter limiters[4];    // global, interface, global volume fraction, interface volume fraction
@@ -90,11 +102,15 @@ class Input {
   static CaseDescription readInputXML(const std::string& caseDir, const std::string& eosDir, int precision);
 };
 
+class Output;
+
 // Run: owns the device context of one case and the reference's time loop
-// (Run::initialize / Run::solver / Run::finalize, Run.cpp:82-498).
+// (Run::initialize / Run::solver / Run::finalize, Run.cpp:82-498).  With an
+// output folder it writes the reference's result files at the case's print
+// times (results/<run>/ under workFolder) and can resume from them.
 class Run {
  public:
-  explicit Run(const CaseDescription& c);
+  explicit Run(const CaseDescription& c, const std::string& workFolder = "", bool writeOutput = false);
   ~Run();
   Run(const Run&) = delete;
   Run& operator=(const Run&) = delete;
@@ -103,10 +119,19 @@ class Run {
   std::vector<std::vector<double>> state() const;
   ecg_ctx* context() const { return m_ctx; }
   const CaseDescription& description() const { return m_case; }
+  long long iteration() const { return m_iteration; }
+  double physicalTime() const { return m_time; }
 
  private:
+  void print(double seconds);
+
   CaseDescription m_case;
+  std::string m_work;
+  bool m_write = false;
   ecg_ctx* m_ctx = nullptr;
+  std::unique_ptr<Output> m_out;
+  long long m_iteration = 0;
+  double m_time = 0., m_dtNext = 0.;
 };
 
 }  // namespace ecg_host

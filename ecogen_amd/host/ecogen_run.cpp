@@ -1,12 +1,14 @@
 // ecogen_run: runs a case written in the reference's XML input format on the
-// GPU through libecogen_amd (Run::initialize + Run::solver, Run.cpp:82-498) and
-// writes the final persistent state as a raw SoA file.
+// GPU through libecogen_amd (Run::initialize + Run::solver, Run.cpp:82-498),
+// writing the reference's result files under <work>/results/<run>/ at the
+// case's print times (and resuming from them when main.xml asks for a restart).
 //
 //   ecogen_run <case dir> [--eos-dir DIR] [--precision fp64|fp32|fp32_learned]
-//              [--max-iter N] [--out FILE]
+//              [--max-iter N] [--work DIR] [--no-output] [--out FILE]
 //
-// The output file holds a text header line (nx ny nz nvar t iterations), then
-// nvar arrays of nx*ny*nz little-endian doubles in the C ABI's variable order.
+// --out also writes the final state as a text header line (nx ny nz nvar t
+// iterations) followed by nvar arrays of nx*ny*nz little-endian doubles in the
+// C ABI's variable order.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -18,16 +20,19 @@
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: %s <case dir> [--eos-dir DIR] [--precision fp64|fp32|fp32_learned] "
-                         "[--max-iter N] [--out FILE]\n", argv[0]);
+                         "[--max-iter N] [--work DIR] [--no-output] [--out FILE]\n", argv[0]);
     return 2;
   }
-  std::string caseDir = argv[1], eosDir, out;
+  std::string caseDir = argv[1], eosDir, out, work = ".";
+  bool write = true;
   int precision = ECG_FP64;
   long long maxIter = -1;
   for (int i = 2; i < argc; i++) {
     const std::string a = argv[i];
     if (a == "--eos-dir" && i + 1 < argc) eosDir = argv[++i];
     else if (a == "--out" && i + 1 < argc) out = argv[++i];
+    else if (a == "--work" && i + 1 < argc) work = argv[++i];
+    else if (a == "--no-output") write = false;
     else if (a == "--max-iter" && i + 1 < argc) maxIter = std::atoll(argv[++i]);
     else if (a == "--precision" && i + 1 < argc) {
       const std::string p = argv[++i];
@@ -45,11 +50,12 @@ int main(int argc, char** argv) {
     if (maxIter > 0) {
       c.control.iterationsMode = true;
       c.control.iterations = maxIter;
+      if (c.control.iterFreq <= 0) c.control.iterFreq = (int)maxIter;
     }
     std::printf("case %s: %s, %d phase(s), %dx%dx%d cells, %s, CFL %g, %zu domains\n", c.control.runName.c_str(),
                 c.model.whoAmI().c_str(), c.model.numberPhases, c.config.nx, c.config.ny, c.config.nz,
                 c.order.c_str(), c.config.cfl, c.domains.size());
-    ecg_host::Run run(c);
+    ecg_host::Run run(c, work, write);
     run.initialize();
     const auto t0 = std::chrono::steady_clock::now();
     const ecg_step_stats st = run.solver();

@@ -1,0 +1,850 @@
+// Result files in the reference's formats; see output.hpp.
+#include "output.hpp"
+
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+#include <sys/stat.h>
+
+#include "xml.hpp"
+
+namespace ecg_host {
+
+namespace {
+
+std::string join(const std::string& a, const std::string& b) {
+  if (a.empty()) return b;
+  return (a.back() == '/') ? a + b : a + "/" + b;
+}
+
+void make_dir(const std::string& p) {
+  if (mkdir(p.c_str(), S_IRWXU) != 0 && errno != EEXIST)
+    throw InputError(ECG_E_ARG, "cannot create the folder " + p + ": " + std::strerror(errno));
+}
+
+void open_or_throw(std::ofstream& f, const std::string& path, std::ios::openmode mode = std::ios::out | std::ios::trunc) {
+  f.open(path.c_str(), mode);
+  if (!f) throw InputError(ECG_E_ARG, "Impossible d ouvrir le file " + path);
+}
+
+// Cell sizes and centres of a uniform axis (MeshCartesian::meshStretching, MeshCartesian.cpp:168-205).
+struct Axis {
+  int n = 1;
+  double d = 0.;
+  double pos(int i) const { return (i + 0.5) * d; }
+};
+
+Axis axis(int n, double l) {
+  Axis a;
+  a.n = n;
+  a.d = l / static_cast<double>(n);
+  return a;
+}
+
+int problem_dimension(const ecg_config& g) { return 1 + (g.ny > 1 ? 1 : 0) + (g.nz > 1 ? 1 : 0); }
+
+const Eos& phase_eos(const CaseDescription& c, int k) { return c.eos[(size_t)k]; }
+
+double eos_temperature(const Eos& e, double eps, double rho, double p) {
+  // EosSG::computeTemperature / EosIG::computeTemperature (EosSG.cpp:66-69, EosIG.cpp:64-67).
+  if (e.type == ECG_EOS_SG) return (p + e.pInf) / (e.gamma - 1.) / std::max(rho, eps) / e.cv;
+  return p / (e.gamma - 1.) / std::max(rho, eps) / e.cv;
+}
+
+double eos_energy(const Eos& e, double eps, double rho, double p) {
+  // EosSG::computeEnergy / EosIG::computeEnergy (EosSG.cpp:73-76, EosIG.cpp:71-74).
+  if (e.type == ECG_EOS_SG) return (p + e.gamma * e.pInf) / (e.gamma - 1.) / std::max(rho, eps) + e.energyRef;
+  return p / (e.gamma - 1.) / std::max(rho, eps) + e.energyRef;
+}
+
+double epsilon(const CaseDescription& c) {
+  return (c.config.model != ECG_EULER && c.config.alpha_null) ? 1.e-15 : 0.;   // Eos.cpp:88-97
+}
+
+std::string eos_stem(const std::string& n) {
+  return (n.size() >= 4 && n.compare(n.size() - 4, 4, ".xml") == 0) ? n.substr(0, n.size() - 4) : n;
+}
+
+// OutputGNU::formatVarNameStyle (OutputGNU.cpp:263-279).
+std::string gnu_style(const std::string& in) {
+  std::string s = in;
+  size_t pos = 0;
+  while ((pos = s.find('_', pos)) != std::string::npos) {
+    s.replace(pos, 1, "\\_");
+    pos += 2;
+  }
+  return eos_stem(s);
+}
+
+// Number of velocity components stored (active axes) and their state offset.
+int state_velocity_offset(const CaseDescription& c) {
+  return c.config.model == ECG_EULER ? 2 : 3 * c.config.nph;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Printed variables
+
+std::vector<double> printedDensity(const CaseDescription& c, const State& s) {
+  const ecg_config& g = c.config;
+  const size_t n = s.empty() ? 0 : s[0].size();
+  std::vector<double> rho(n);
+  if (g.model == ECG_EULER) {
+    rho = s[0];
+    return rho;
+  }
+  for (size_t i = 0; i < n; i++) {
+    double r = 0.;
+    for (int k = 0; k < g.nph; k++) r += s[3 * k][i] * s[3 * k + 1][i];   // MixUEq::computeMixtureVariables
+    rho[i] = r;
+  }
+  return rho;
+}
+
+std::vector<PrintedVariable> printedVariables(const CaseDescription& c, const State& s) {
+  const ecg_config& g = c.config;
+  const size_t n = s.empty() ? 0 : s[0].size();
+  const int D = problem_dimension(g);
+  const int uo = state_velocity_offset(c);
+  const double eps = epsilon(c);
+  std::vector<PrintedVariable> out;
+  auto velocity = [&](const std::string& name, int owner) {
+    PrintedVariable v;
+    v.name = name;
+    v.owner = owner;
+    v.vector = true;
+    v.v.assign(3 * n, 0.);
+    for (size_t i = 0; i < n; i++)
+      for (int a = 0; a < D; a++) v.v[3 * i + a] = s[(size_t)(uo + a)][i];
+    return v;
+  };
+  auto scalar = [&](const std::string& name, int owner) {
+    PrintedVariable v;
+    v.name = name;
+    v.owner = owner;
+    v.v.assign(n, 0.);
+    return v;
+  };
+  if (g.model == ECG_EULER) {
+    // PhaseEuler::returnScalar / returnVector (PhaseEuler.cpp:131-185).
+    const Eos& e = phase_eos(c, 0);
+    PrintedVariable rho = scalar("Density", 0), p = scalar("Pressure", 0), T = scalar("Temperature", 0);
+    for (size_t i = 0; i < n; i++) {
+      rho.v[i] = s[0][i];
+      p.v[i] = s[1][i];
+      T.v[i] = eos_temperature(e, eps, s[0][i], s[1][i]);
+    }
+    out.push_back(std::move(rho));
+    out.push_back(std::move(p));
+    out.push_back(std::move(T));
+    out.push_back(velocity("Velocity", 0));
+    return out;
+  }
+  const bool ueq = (g.model == ECG_UEQ);
+  const bool reduced = c.output.reducedOutput;
+  const std::vector<double> rhoMix = printedDensity(c, s);
+  std::vector<double> pMix(n, 0.);
+  for (size_t i = 0; i < n; i++) {
+    double p = 0.;
+    for (int k = 0; k < g.nph; k++) p += s[3 * k][i] * s[3 * k + 2][i];
+    pMix[i] = p;
+  }
+  for (int k = 0; k < g.nph; k++) {
+    // PhaseUEq::returnScalar (PhaseUEq.cpp:115-141), PhasePUEq::returnScalar (PhasePUEq.cpp:87-110);
+    // reducedOutput keeps the first 3 (UEq) or 2 (PUEq) of them (Input.cpp:601-631).
+    const Eos& e = phase_eos(c, k);
+    const std::vector<double>& a = s[(size_t)(3 * k)];
+    const std::vector<double>& r = s[(size_t)(3 * k + 1)];
+    const std::vector<double>& p = s[(size_t)(3 * k + 2)];
+    PrintedVariable va = scalar("Alpha", k), vr = scalar("Density", k), vp = scalar("Pressure", k);
+    PrintedVariable vT = scalar("Temperature", k), vY = scalar("Mass fraction", k);
+    for (size_t i = 0; i < n; i++) {
+      va.v[i] = (a[i] < 1.e-20) ? 0. : a[i];
+      vr.v[i] = (r[i] < 1.e-20) ? 1.e-20 : r[i];
+      vp.v[i] = p[i];
+      const double T = eos_temperature(e, eps, r[i], p[i]);
+      vT.v[i] = (T < 1.e-20) ? 1.e-20 : T;
+      const double Y = a[i] * r[i] / std::max(rhoMix[i], eps);   // PhaseUEq::computeMassFraction
+      vY.v[i] = (Y < 1.e-20) ? 0. : Y;
+    }
+    out.push_back(std::move(va));
+    out.push_back(std::move(vr));
+    if (ueq) out.push_back(std::move(vp));
+    if (!reduced) {
+      out.push_back(std::move(vT));
+      out.push_back(std::move(vY));
+    }
+  }
+  // MixUEq::returnScalar / returnVector (MixUEq.cpp:201-259).
+  PrintedVariable dm = scalar("Density_Mixture", -1), pm = scalar("Pressure_Mixture", -1);
+  PrintedVariable em = scalar("Total_energy_Mixture", -1);
+  const std::vector<double>& E = s[(size_t)(uo + D)];
+  for (size_t i = 0; i < n; i++) {
+    dm.v[i] = rhoMix[i];
+    pm.v[i] = pMix[i];
+    em.v[i] = E[i];
+  }
+  out.push_back(std::move(dm));
+  out.push_back(std::move(pm));
+  if (!reduced) out.push_back(std::move(em));
+  out.push_back(velocity("Velocity_Mixture", -1));
+  return out;
+}
+
+std::vector<double> densityGradientNorm(const CaseDescription& c, const std::vector<double>& rho) {
+  // GradientFiniteDifference::computeGradients over the cell's interfaces in
+  // creation order (inner faces X, Y, Z, then boundary faces X-, X+, Y-, Y+, Z-, Z+,
+  // MeshCartesian.cpp:283-557); boundary faces only add their distances for a
+  // density gradient.  Positions as the mesh stores them.
+  const ecg_config& g = c.config;
+  const Axis ax = axis(g.nx, g.lx), ay = axis(g.ny, g.ly), az = axis(g.nz, g.lz);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  std::vector<double> out(rho.size(), 0.);
+  auto idx = [&](int i, int j, int k) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k); };
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        double gx = 0., gy = 0., gz = 0., sx = 0., sy = 0., sz = 0.;
+        const double x = ax.pos(i), y = ay.pos(j), z = az.pos(k);
+        auto inner = [&](int il, int jl, int kl, int ir, int jr, int kr, int a) {
+          const double xl = ax.pos(il), yl = ay.pos(jl), zl = az.pos(kl);
+          const double xr = ax.pos(ir), yr = ay.pos(jr), zr = az.pos(kr);
+          const double vx = xl - xr, vy = yl - yr, vz = zl - zr;
+          const double dX = std::fabs(vx), dY = std::fabs(vy), dZ = std::fabs(vz);
+          sx += dX; sy += dY; sz += dZ;
+          const double dist = std::sqrt(vx * vx + vy * vy + vz * vz);
+          const double gf = (rho[idx(ir, jr, kr)] - rho[idx(il, jl, kl)]) / dist;
+          const double nxv = (a == 0) ? 1. : 0., nyv = (a == 1) ? 1. : 0., nzv = (a == 2) ? 1. : 0.;
+          gx = gx + nxv * gf * dX;
+          gy = gy + nyv * gf * dY;
+          gz = gz + nzv * gf * dZ;
+        };
+        auto boundary = [&](double fx, double fy, double fz) {
+          sx += std::fabs(x - fx) * 2.;
+          sy += std::fabs(y - fy) * 2.;
+          sz += std::fabs(z - fz) * 2.;
+        };
+        if (i > 0) inner(i - 1, j, k, i, j, k, 0);
+        if (i < nx - 1) inner(i, j, k, i + 1, j, k, 0);
+        if (j > 0) inner(i, j - 1, k, i, j, k, 1);
+        if (j < ny - 1) inner(i, j, k, i, j + 1, k, 1);
+        if (k > 0) inner(i, j, k - 1, i, j, k, 2);
+        if (k < nz - 1) inner(i, j, k, i, j, k + 1, 2);
+        if (i == 0) boundary(0., y, z);
+        if (i == nx - 1) boundary(x + 0.5 * ax.d, y, z);
+        if (ny > 1) {
+          if (j == 0) boundary(x, 0., z);
+          if (j == ny - 1) boundary(x, y + 0.5 * ay.d, z);
+        }
+        if (nz > 1) {
+          if (k == 0) boundary(x, y, 0.);
+          if (k == nz - 1) boundary(x, y, z + 0.5 * az.d);
+        }
+        if (sx <= 1.e-12) sx = 1.;
+        if (sy <= 1.e-12) sy = 1.;
+        if (sz <= 1.e-12) sz = 1.;
+        gx = gx / sx;
+        gy = gy / sy;
+        gz = gz / sz;
+        out[idx(i, j, k)] = std::sqrt(gx * gx + gy * gy + gz * gz);
+      }
+  return out;
+}
+
+State stateFromPrinted(const CaseDescription& c, const std::vector<PrintedVariable>& pv) {
+  const ecg_config& g = c.config;
+  const int D = problem_dimension(g);
+  const size_t n = pv.empty() ? 0 : pv[0].v.size();
+  const double eps = epsilon(c);
+  auto find = [&](const std::string& name, int owner) -> const std::vector<double>* {
+    for (const auto& v : pv)
+      if (v.name == name && v.owner == owner) return &v.v;
+    return nullptr;
+  };
+  auto velocity = [&](int owner, const char* name, State& st, int uo) {
+    const std::vector<double>* u = find(name, owner);
+    if (!u) throw InputError(ECG_E_ARG, std::string("restart file lacks ") + name);
+    for (int a = 0; a < D; a++)
+      for (size_t i = 0; i < n; i++) st[(size_t)(uo + a)][i] = (*u)[3 * i + a];
+    return u;
+  };
+  if (g.model == ECG_EULER) {
+    State st((size_t)(D + 3), std::vector<double>(n));
+    st[0] = *find("Density", 0);
+    st[1] = *find("Pressure", 0);
+    const std::vector<double>* u = velocity(0, "Velocity", st, 2);
+    // PhaseEuler: E = e(rho, p) + |u|^2 / 2 (Phase::computeTotalEnergy via completeFulfillState).
+    const Eos& e = phase_eos(c, 0);
+    for (size_t i = 0; i < n; i++) {
+      const double ux = (*u)[3 * i], uy = (*u)[3 * i + 1], uz = (*u)[3 * i + 2];
+      st[(size_t)(2 + D)][i] = eos_energy(e, eps, st[0][i], st[1][i]) + 0.5 * (ux * ux + uy * uy + uz * uz);
+    }
+    return st;
+  }
+  const int N = g.nph;
+  State st((size_t)(3 * N + D + 1), std::vector<double>(n));
+  const std::vector<double>* pm = find("Pressure_Mixture", -1);
+  if (!pm) throw InputError(ECG_E_ARG, "restart file lacks Pressure_Mixture");
+  for (int k = 0; k < N; k++) {
+    const std::vector<double>* a = find("Alpha", k);
+    const std::vector<double>* r = find("Density", k);
+    const std::vector<double>* p = (g.model == ECG_UEQ) ? find("Pressure", k) : pm;
+    if (!a || !r || !p) throw InputError(ECG_E_ARG, "restart file lacks phase variables");
+    st[(size_t)(3 * k)] = *a;
+    st[(size_t)(3 * k + 1)] = *r;
+    st[(size_t)(3 * k + 2)] = *p;
+  }
+  if (g.model == ECG_PUEQ) {
+    // ModPUEq::fulfillStateRestart (ModPUEq.cpp:76-82): phase pressure from the
+    // mixture, then PhaseUEq::verifyAndCorrectPhase (PhaseUEq.cpp:355-368).
+    for (int k = 0; k < N; k++)
+      for (size_t i = 0; i < n; i++) {
+        double& a = st[(size_t)(3 * k)][i];
+        double& r = st[(size_t)(3 * k + 1)][i];
+        double& p = st[(size_t)(3 * k + 2)][i];
+        if (eps > 1.e-20) {
+          if (a < 0.) a = 0.;
+          if (a > 1.) a = 1.;
+          if (r <= 1.e-15) r = 1.e-15;
+        } else {
+          if (a < 1e-15) a = 1e-14;
+          if (a > 1. - 1e-15) a = 1. - 1e-14;
+          if (r < 1.e-15) r = 1.e-14;
+        }
+        const Eos& e = phase_eos(c, k);
+        if (e.type == ECG_EOS_SG) {
+          // EosSG::verifyAndModifyPressure (EosSG.cpp:253-256)
+          if (p <= -(1. - 1.e-15) * e.pInf + 1.e-15) p = -(1. - 1.e-15) * e.pInf + 1.e-15;
+        } else if (p < 1.e-8) {
+          p = 1.e-8;   // EosIG::verifyAndModifyPressure (EosIG.cpp:251-254)
+        }
+      }
+  }
+  const std::vector<double>* u = velocity(-1, "Velocity_Mixture", st, 3 * N);
+  // Cell::completeFulfillState: mixture energy sum_k Y_k e_k, then E = e + |u|^2 / 2
+  // (MixUEq::computeMixtureVariables, MixUEq::computeTotalEnergy, MixUEq.cpp:136-171).
+  for (size_t i = 0; i < n; i++) {
+    double rho = 0.;
+    for (int k = 0; k < N; k++) rho += st[(size_t)(3 * k)][i] * st[(size_t)(3 * k + 1)][i];
+    double em = 0.;
+    for (int k = 0; k < N; k++) {
+      const double a = st[(size_t)(3 * k)][i], r = st[(size_t)(3 * k + 1)][i], p = st[(size_t)(3 * k + 2)][i];
+      const double Y = a * r / std::max(rho, eps);
+      em += Y * eos_energy(phase_eos(c, k), eps, r, p);
+    }
+    const double ux = (*u)[3 * i], uy = (*u)[3 * i + 1], uz = (*u)[3 * i + 2];
+    st[(size_t)(3 * N + D)][i] = em + 0.5 * (ux * ux + uy * uy + uz * uz);
+  }
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// Base64 (standard alphabet, '=' padding), byte for byte what IO::writeb64Chaine produces.
+
+std::string base64(const unsigned char* d, size_t n) {
+  static const char* A = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+  std::string out;
+  out.reserve((n + 2) / 3 * 4);
+  size_t i = 0;
+  for (; i + 3 <= n; i += 3) {
+    const unsigned v = ((unsigned)d[i] << 16) | ((unsigned)d[i + 1] << 8) | d[i + 2];
+    out += A[(v >> 18) & 63];
+    out += A[(v >> 12) & 63];
+    out += A[(v >> 6) & 63];
+    out += A[v & 63];
+  }
+  const size_t r = n - i;
+  if (r) {
+    const unsigned v = ((unsigned)d[i] << 16) | ((r > 1 ? (unsigned)d[i + 1] : 0u) << 8);
+    out += A[(v >> 18) & 63];
+    out += A[(v >> 12) & 63];
+    out += (r > 1) ? A[(v >> 6) & 63] : '=';
+    out += '=';
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Output
+
+Output::Output(const CaseDescription& c, const std::string& workFolder) : m_case(c) {
+  // Output::Output (Output.cpp:43-111): results/<run>/ and its sub-folders.
+  const std::string results = join(workFolder, "results/");
+  m_folderOutput = results + c.control.runName + "/";
+  m_folderDatasets = m_folderOutput + "datasets/";
+  m_folderInfoMesh = m_folderOutput + "infoMesh/";
+  m_folderGlobal = m_folderOutput + "globalQuantities/";
+  make_dir(results);
+  make_dir(m_folderOutput);
+  for (const char* sub : {"savesInput/", "datasets/", "infoMesh/", "cuts/", "probes/", "globalQuantities/",
+                          "boundaries/", "errorsAndWarnings/"})
+    make_dir(m_folderOutput + sub);
+  m_dim = problem_dimension(c.config);
+}
+
+std::unique_ptr<Output> Output::create(const CaseDescription& c, const std::string& workFolder) {
+  if (c.output.format == "GNU") return std::unique_ptr<Output>(new OutputGNU(c, workFolder));
+  return std::unique_ptr<Output>(new OutputXML(c, workFolder));
+}
+
+void Output::setPrecision(std::ofstream& f) const {
+  if (m_case.output.precision != 0) f.precision(m_case.output.precision);
+}
+
+std::string Output::createFilename(const std::string& name, int proc, int number) const {
+  std::ostringstream s;
+  s << name;
+  if (proc > -1) s << "_CPU" << proc;
+  if (number != -1) s << "_TIME" << number;
+  s << ".out";
+  return s.str();
+}
+
+void Output::copyInputFiles() const {
+  for (const char* f : {"main.xml", "mesh.xml", "initialConditions.xml", "model.xml"}) {
+    std::ifstream src(join(m_case.caseDir, f).c_str());
+    if (!src) throw InputError(ECG_E_ARG, std::string("file not found \"") + join(m_case.caseDir, f) + "\"");
+    std::ofstream dst;
+    open_or_throw(dst, m_folderOutput + "savesInput/" + f);
+    std::string line;
+    while (std::getline(src, line)) dst << line << std::endl;
+  }
+}
+
+void Output::initializeOutput() {
+  if (auto* x = dynamic_cast<OutputXML*>(this)) x->initializeCollections();
+  for (const std::string& q : m_case.output.globalQuantities)
+    initializeGlobalQuantity(m_case, m_folderGlobal, q, m_case.output.restartFileNumber > 0);
+}
+
+void Output::initializeOutputInfos() {
+  std::ofstream f;
+  open_or_throw(f, m_folderOutput + "infoCalcul.out");
+  f.close();
+  open_or_throw(f, m_folderInfoMesh + createFilename("infoMesh", 0, -1));
+}
+
+void Output::writeInfos(long long iteration, double t, double dtNext, double seconds) {
+  std::ofstream f;
+  setPrecision(f);
+  open_or_throw(f, m_folderOutput + "infoCalcul.out", std::ios::out | std::ios::app);
+  if (m_numFichier == 0) f << 1 << std::endl;
+  f << m_numFichier << " " << iteration << " " << t << " " << dtNext << " " << seconds << " " << 0. << " " << 0.;
+  f << std::endl;
+}
+
+void Output::saveInfoCells(long long ncell) {
+  std::ofstream f;
+  open_or_throw(f, m_folderInfoMesh + createFilename("infoMesh", 0, -1), std::ios::out | std::ios::app);
+  f << ncell << std::endl;
+}
+
+void Output::writeGlobalQuantities(double t, const State& s) {
+  for (const std::string& q : m_case.output.globalQuantities) {
+    const double v = globalQuantity(m_case, s, q);
+    std::ofstream f;
+    setPrecision(f);
+    open_or_throw(f, m_folderGlobal + q + ".out", std::ios::out | std::ios::app);
+    f << t << " " << v << std::endl;
+  }
+}
+
+void Output::prepareErrorFiles() const {
+  std::ofstream f;
+  open_or_throw(f, m_folderOutput + "errorsAndWarnings/error_CPU0.out");
+  f.close();
+  open_or_throw(f, m_folderOutput + "errorsAndWarnings/warning_CPU0.out");
+}
+
+void Output::writeDataset(std::ofstream& f, const std::vector<double>& v) const {
+  // Output::writeDataset (Output.cpp:376-428), Float64 only.
+  setPrecision(f);
+  if (!m_case.output.binary) {
+    for (double x : v) f << x << " ";
+    return;
+  }
+  const int size = (int)(v.size() * sizeof(double));
+  f << base64(reinterpret_cast<const unsigned char*>(&size), sizeof(size));
+  f << base64(reinterpret_cast<const unsigned char*>(v.data()), v.size() * sizeof(double));
+}
+
+void Output::readInfos(int number, long long& iteration, double& t, double& dt) {
+  const std::string path = m_folderOutput + "infoCalcul.out";
+  std::ifstream f(path.c_str());
+  if (!f) throw InputError(ECG_E_ARG, "restart simulation not possible - check file 'infosCalcul.out'");
+  std::vector<std::string> keep;
+  std::string line;
+  if (!std::getline(f, line)) throw InputError(ECG_E_ARG, "restart simulation not possible - check file 'infosCalcul.out'");
+  if (std::stoi(line) != 1) throw InputError(ECG_E_ARG, "restart simulation not possible - number of CPU differs from read files");
+  keep.push_back(line);
+  bool found = false;
+  while (std::getline(f, line)) {
+    std::istringstream ls(line);
+    int num = -1;
+    double c0, c1, c2;
+    ls >> num >> iteration >> t >> dt >> c0 >> c1 >> c2;
+    keep.push_back(line);
+    if (num == number) { found = true; break; }
+  }
+  if (!found) throw InputError(ECG_E_ARG, "restart simulation not possible - check file 'infosCalcul.out'");
+  f.close();
+  std::ofstream o;
+  open_or_throw(o, path);
+  for (const auto& l : keep) o << l << std::endl;
+  m_numFichier = number;
+}
+
+State Output::readResults() {
+  throw InputError(ECG_E_UNSUPPORTED, "Run::restartSimulation: Restart option only available for XML output");
+}
+
+// ---------------------------------------------------------------------------
+// GNU
+
+void OutputGNU::writeResults(const State& s) {
+  // OutputGNU::writeResults (OutputGNU.cpp:61-78) through Cell::printGnuplotAMR (Cell.cpp:759-792).
+  const std::vector<PrintedVariable> pv = printedVariables(m_case, s);
+  const ecg_config& g = m_case.config;
+  const Axis ax = axis(g.nx, g.lx), ay = axis(g.ny, g.ly), az = axis(g.nz, g.lz);
+  std::ofstream f;
+  open_or_throw(f, m_folderDatasets + createFilename("result", 0, m_numFichier));
+  setPrecision(f);
+  const size_t n = s[0].size();
+  for (size_t c = 0; c < n; c++) {
+    const int i = (int)(c % (size_t)g.nx), j = (int)((c / (size_t)g.nx) % (size_t)g.ny);
+    const int k = (int)(c / ((size_t)g.nx * (size_t)g.ny));
+    if (m_dim >= 1) f << ax.pos(i) << " ";
+    if (m_dim >= 2) f << ay.pos(j) << " ";
+    if (m_dim == 3) f << az.pos(k) << " ";
+    for (const auto& v : pv) {
+      if (!v.vector) {
+        f << v.v[c] << " ";
+      } else {
+        const double x = v.v[3 * c], y = v.v[3 * c + 1], z = v.v[3 * c + 2];
+        f << std::sqrt(x * x + y * y + z * z) << " ";   // Coord::norm
+      }
+    }
+    f << 0 << " " << 0. << " ";   // AMR level and xi of a Cartesian mesh
+    f << std::endl;
+  }
+  f << std::endl;
+  f.close();
+  writeScriptGnuplot(pv);
+  m_numFichier++;
+}
+
+void OutputGNU::writeScriptGnuplot(const std::vector<PrintedVariable>& pv) const {
+  // OutputGNU::writeScriptGnuplot and printBlocGnuplot (OutputGNU.cpp:82-226).
+  std::ofstream f;
+  open_or_throw(f, m_folderOutput + "visualization.gnu");
+  f << "reset" << std::endl << "set style data lines" << std::endl;
+  f << "set nokey" << std::endl;
+  f << std::endl;
+  int index = 1 + m_dim;
+  f << "set xlabel 'x (m)'" << std::endl;
+  if (m_dim == 2) {
+    f << "set surface" << std::endl;
+    f << "set dgrid3d 50,50" << std::endl;
+    f << "set contour base" << std::endl;
+    f << "set cntrparam levels 25" << std::endl;
+    f << "show contour" << std::endl;
+  }
+  auto block = [&](const std::string& title) {
+    f << "set title '" << title << "'" << std::endl;
+    f << (m_dim <= 1 ? "plot" : "splot");
+    for (int t = 0; t <= m_numFichier; t++) {
+      f << " \"" << "./datasets/" << createFilename("result", 0, t) << "\"";
+      if (m_dim <= 1) f << " u 1:" << index;
+      else f << " u 1:2:" << index;
+      if (t != m_numFichier) f << ",\\";
+      f << std::endl;
+    }
+    f << "pause(-1)" << std::endl << std::endl;
+    index++;
+  };
+  for (const auto& v : pv) {
+    if (v.owner >= 0) block(gnu_style(v.name) + "\\_" + gnu_style(phase_eos(m_case, v.owner).name));
+    else block(gnu_style(v.name));
+  }
+  block("AMR level");
+  block("Xi");
+}
+
+// ---------------------------------------------------------------------------
+// XML (VTK rectilinear grid)
+
+std::string OutputXML::datasetName(int number) const {
+  std::ostringstream s;
+  s << "result";
+  if (m_case.output.binary) s << "B64";
+  s << "_CPU0_TIME" << number << ".vtr";
+  return s.str();
+}
+
+void OutputXML::initializeCollections() const {
+  // OutputXML::initializeSpecificOutput (OutputXML.cpp:58-83).
+  const std::string b64 = m_case.output.binary ? "B64" : "";
+  std::ofstream f;
+  open_or_throw(f, m_folderOutput + "collectionParaview" + b64 + ".pvd");
+  f << "<?xml version=\"1.0\"?>" << std::endl;
+  f << "<VTKFile type=\"Collection\" version=\"0.1\" byte_order=\"LittleEndian\" ";
+  f << "compressor=\"vtkZLibDataCompressor\">";
+  f << std::endl << "  <Collection>" << std::endl;
+  f << std::endl << "  </Collection>" << std::endl;
+  f << "</VTKFile>" << std::endl;
+  f.close();
+  open_or_throw(f, m_folderOutput + "collectionVisIt" + b64 + ".visit");
+  f << "!NBLOCKS " << 1 << std::endl;
+}
+
+void OutputXML::writeCollections() const {
+  // OutputXML::writeCollectionXML (OutputXML.cpp:371-414).
+  const std::string b64 = m_case.output.binary ? "B64" : "";
+  std::ofstream f;
+  open_or_throw(f, m_folderOutput + "collectionParaview" + b64 + ".pvd");
+  f << "<?xml version=\"1.0\"?>" << std::endl;
+  f << "<VTKFile type=\"Collection\" version=\"0.1\" byte_order=\"LittleEndian\" ";
+  f << "compressor=\"vtkZLibDataCompressor\">";
+  f << std::endl << "    <Collection>" << std::endl;
+  for (int t = 0; t <= m_numFichier; t++)
+    f << "        <DataSet timestep=\"" << t << "\" part=\"" << 0 << "\" file=\"datasets/" << datasetName(t) << "\"/>"
+      << std::endl;
+  f << "    </Collection>" << std::endl;
+  f << "</VTKFile>" << std::endl;
+  f.close();
+  open_or_throw(f, m_folderOutput + "collectionVisIt" + b64 + ".visit");
+  f << "!NBLOCKS " << 1 << std::endl;
+  for (int t = 0; t <= m_numFichier; t++) f << "datasets/" << datasetName(t) << std::endl;
+}
+
+void OutputXML::writeResults(const State& s) {
+  // OutputXML::writeResultsXML, writeMeshRectilinearXML, writePhysicalDataXML,
+  // writeFinFichierRectilinearXML (OutputXML.cpp:320-366, 418-690, 774-783).
+  const ecg_config& g = m_case.config;
+  const std::vector<PrintedVariable> pv = printedVariables(m_case, s);
+  const std::string fmt = m_case.output.binary ? "binary" : "ascii";
+  std::ofstream f;
+  open_or_throw(f, m_folderDatasets + datasetName(m_numFichier));
+  f << "<?xml version=\"1.0\"?>" << std::endl;
+  f << "<VTKFile type=\"RectilinearGrid\" version=\"0.1\" byte_order=\"LittleEndian\">" << std::endl;
+  std::ostringstream ext;
+  ext << 0 << " " << g.nx << " " << 0 << " " << g.ny << " " << 0 << " " << g.nz;
+  f << "  <RectilinearGrid WholeExtent=\"" << ext.str() << "\">" << std::endl;
+  f << "    <Piece Extent=\"" << ext.str() << "\">" << std::endl;
+  f << "      <Coordinates>" << std::endl;
+  const Axis axes[3] = {axis(g.nx, g.lx), axis(g.ny, g.ly), axis(g.nz, g.lz)};
+  for (const Axis& a : axes) {
+    // MeshCartesian::getCoord (MeshCartesian.cpp:1791-1816): cell faces.
+    std::vector<double> x;
+    for (int i = 0; i < a.n; i++) x.push_back(a.pos(i) - 0.5 * a.d);
+    x.push_back(a.pos(a.n - 1) + 0.5 * a.d);
+    f << "        <DataArray type=\"Float64\" ";
+    if (!m_case.output.binary) f << "format=\"ascii\">" << std::endl << "          ";
+    else f << "format=\"binary\">" << std::endl;
+    writeDataset(f, x);
+    f << std::endl;
+    f << "        </DataArray>" << std::endl;
+  }
+  f << "      </Coordinates>" << std::endl;
+  f << "      <CellData>" << std::endl;
+  const bool multiphase = g.model != ECG_EULER;
+  for (const auto& v : pv) {
+    if (v.owner < 0 && !multiphase) continue;
+    f << "        <DataArray type=\"Float64\" Name=\"";
+    if (v.owner >= 0) f << "F" << v.owner << "_" << v.name << "_" << eos_stem(phase_eos(m_case, v.owner).name);
+    else f << v.name;
+    f << "\"";
+    if (v.vector) f << " NumberOfComponents=\"3\"";
+    f << " format=\"" << fmt << "\">" << std::endl;
+    writeDataset(f, v.v);
+    f << std::endl;
+    f << "        </DataArray>" << std::endl;
+  }
+  f << "        <DataArray type=\"Float64\" Name=\"gradRho\"";
+  f << " format=\"" << fmt << "\">" << std::endl;
+  writeDataset(f, densityGradientNorm(m_case, printedDensity(m_case, s)));
+  f << std::endl;
+  f << "        </DataArray>" << std::endl;
+  f << "      </CellData>" << std::endl;
+  f << "    </Piece>" << std::endl;
+  f << "  </RectilinearGrid>" << std::endl;
+  f << "</VTKFile>" << std::endl;
+  f.close();
+  writeCollections();
+  m_numFichier++;
+}
+
+State OutputXML::readResults() {
+  // OutputXML::readResults / ReadPhysicalDataXML (OutputXML.cpp:101-270): the
+  // arrays are taken in print order from the CellData element, ASCII only.
+  if (m_case.output.binary) throw InputError(ECG_E_UNSUPPORTED, "resuming on binary results file not available");
+  const std::string path = m_folderDatasets + datasetName(m_numFichier);
+  std::unique_ptr<XmlElement> root;
+  try {
+    root = load_xml(path);
+  } catch (const XmlError& e) {
+    throw InputError(ECG_E_ARG, e.what());
+  }
+  if (root->name != "VTKFile") throw InputError(ECG_E_ARG, path + ": root element must be <VTKFile>");
+  const XmlElement* grid = root->child("RectilinearGrid");
+  const XmlElement* piece = grid ? grid->child("Piece") : nullptr;
+  const XmlElement* cd = piece ? piece->child("CellData") : nullptr;
+  if (!cd) throw InputError(ECG_E_ARG, path + ": no RectilinearGrid/Piece/CellData");
+  const std::vector<const XmlElement*> arrays = cd->children_named("DataArray");
+  const ecg_config& g = m_case.config;
+  const size_t n = (size_t)g.nx * g.ny * g.nz;
+  // The layout expected is the one this case prints.
+  State zero((size_t)(g.model == ECG_EULER ? problem_dimension(g) + 3 : 3 * g.nph + problem_dimension(g) + 1),
+             std::vector<double>(n, 1.));
+  std::vector<PrintedVariable> pv = printedVariables(m_case, zero);
+  size_t q = 0;
+  for (auto& v : pv) {
+    if (v.owner < 0 && g.model == ECG_EULER) continue;
+    if (q >= arrays.size()) throw InputError(ECG_E_ARG, path + ": fewer DataArray elements than the case prints");
+    std::istringstream data(arrays[q]->text);
+    for (double& x : v.v) {
+      if (!(data >> x)) throw InputError(ECG_E_ARG, path + ": DataArray " + std::to_string(q) + " is too short");
+    }
+    q++;
+  }
+  m_numFichier++;
+  return stateFromPrinted(m_case, pv);
+}
+
+// ---------------------------------------------------------------------------
+// Global quantities
+
+void initializeGlobalQuantity(const CaseDescription& c, const std::string& folder, const std::string& quantity,
+                              bool restarted) {
+  // OutputGlobalGNU::initializeSpecificOutput and OutputGNU::writeScriptGnuplot(varName)
+  // (OutputGlobalGNU.cpp:68-86, OutputGNU.cpp:178-199).
+  (void)c;
+  std::ofstream f;
+  open_or_throw(f, folder + quantity + ".out",
+                restarted ? (std::ios::out | std::ios::app) : (std::ios::out | std::ios::trunc));
+  f.close();
+  open_or_throw(f, folder + "plot_" + quantity + ".gnu");
+  f << "reset" << std::endl << "set style data lines" << std::endl;
+  f << "set nokey" << std::endl;
+  f << std::endl;
+  f << "set xlabel 't (s)'" << std::endl;
+  f << "set title '" << gnu_style(quantity) << "'" << std::endl;
+  f << "plot" << " \"" << quantity << ".out" << "\"" << " u 1:" << 2 << std::endl;
+  f << "pause(-1)" << std::endl << std::endl;
+}
+
+double globalQuantity(const CaseDescription& c, const State& s, const std::string& quantity) {
+  // Cell::computeTotalMass / computeTotalEnergy (Cell.cpp:863-895), cells in index order.
+  const ecg_config& g = c.config;
+  const double vol = (g.lx / static_cast<double>(g.nx)) * (g.ly / static_cast<double>(g.ny)) *
+                     (g.lz / static_cast<double>(g.nz));
+  const std::vector<double> rho = printedDensity(c, s);
+  const int D = problem_dimension(g);
+  const std::vector<double>& E = s[(size_t)(state_velocity_offset(c) + D)];
+  double q = 0.;
+  if (quantity == "mass") {
+    for (size_t i = 0; i < rho.size(); i++) q += rho[i] * vol;
+  } else {
+    for (size_t i = 0; i < rho.size(); i++) q += rho[i] * E[i] * vol;
+  }
+  return q;
+}
+
+}  // namespace ecg_host
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+namespace ecg {
+int api_fail(int code, const std::string& msg);
+}
+
+struct ecg_output {
+  ecg_host::CaseDescription c;
+  std::unique_ptr<ecg_host::Output> out;
+};
+
+namespace {
+
+template <class F>
+int guarded(F f) {
+  try {
+    return f();
+  } catch (const ecg_host::InputError& e) {
+    return ecg::api_fail(e.code, e.what());
+  } catch (const ecg_host::XmlError& e) {
+    return ecg::api_fail(ECG_E_ARG, e.what());
+  } catch (const std::exception& e) {
+    return ecg::api_fail(ECG_E_ARG, e.what());
+  }
+}
+
+int expected_nvar(const ecg_config& g) {
+  const int D = 1 + (g.ny > 1 ? 1 : 0) + (g.nz > 1 ? 1 : 0);
+  return g.model == ECG_EULER ? D + 3 : 3 * g.nph + D + 1;
+}
+
+}  // namespace
+
+extern "C" ecg_output* ecg_output_open(const char* case_dir, const char* eos_dir, const char* work_dir) {
+  ecg_output* o = nullptr;
+  const int rc = guarded([&] {
+    if (!case_dir || !work_dir) return ecg::api_fail(ECG_E_ARG, "null argument");
+    std::unique_ptr<ecg_output> h(new ecg_output);
+    h->c = ecg_host::Input::readInputXML(case_dir, eos_dir ? eos_dir : "", ECG_FP64);
+    h->out = ecg_host::Output::create(h->c, work_dir);
+    h->out->copyInputFiles();
+    h->out->initializeOutput();
+    if (h->c.output.restartFileNumber == 0) {
+      h->out->initializeOutputInfos();
+      h->out->prepareErrorFiles();
+    }
+    o = h.release();
+    return (int)ECG_OK;
+  });
+  return rc == ECG_OK ? o : nullptr;
+}
+
+extern "C" int ecg_output_write(ecg_output* o, const double* const* soa, int nvars, int64_t iteration, double t,
+                                double dt_next, double seconds) {
+  return guarded([&] {
+    if (!o || !soa) return ecg::api_fail(ECG_E_ARG, "null argument");
+    const ecg_config& g = o->c.config;
+    if (nvars != expected_nvar(g)) return ecg::api_fail(ECG_E_ARG, "expected " + std::to_string(expected_nvar(g)) + " state arrays");
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    ecg_host::State st((size_t)nvars);
+    for (int v = 0; v < nvars; v++) {
+      if (!soa[v]) return ecg::api_fail(ECG_E_ARG, "null state array");
+      st[(size_t)v].assign(soa[v], soa[v] + n);
+    }
+    o->out->writeInfos(iteration, t, dt_next, seconds);
+    o->out->saveInfoCells((long long)n);
+    o->out->writeGlobalQuantities(t, st);
+    o->out->writeResults(st);
+    return (int)ECG_OK;
+  });
+}
+
+extern "C" int ecg_output_restart(ecg_output* o, double* const* soa, int nvars, int64_t* iteration, double* t,
+                                  double* dt) {
+  return guarded([&] {
+    if (!o || !soa || !iteration || !t || !dt) return ecg::api_fail(ECG_E_ARG, "null argument");
+    const int number = o->c.output.restartFileNumber;
+    if (number <= 0) return ecg::api_fail(ECG_E_ARG, "the case asks for no restart (restartFileNumber)");
+    if (nvars != expected_nvar(o->c.config)) return ecg::api_fail(ECG_E_ARG, "wrong number of state arrays");
+    long long it = 0;
+    o->out->readInfos(number, it, *t, *dt);
+    *iteration = it;
+    const ecg_host::State st = o->out->readResults();
+    for (int v = 0; v < nvars; v++) {
+      if (!soa[v]) return ecg::api_fail(ECG_E_ARG, "null state array");
+      std::copy(st[(size_t)v].begin(), st[(size_t)v].end(), soa[v]);
+    }
+    return (int)ECG_OK;
+  });
+}
+
+extern "C" void ecg_output_close(ecg_output* o) { delete o; }

@@ -1,0 +1,103 @@
+// Result files in the reference's formats (src/InputOutput/Output*.cpp): the
+// results/<run>/ folder tree, infoCalcul.out, infoMesh, the GNU column files
+// with their gnuplot script, the VTK XML rectilinear-grid files (ASCII or
+// base64) with their ParaView/VisIt collections, the global-quantity files, and
+// the XML restart reader.  Everything is written on the host from the state the
+// device returns (ecg_download_state layout).
+#pragma once
+
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ecogen.hpp"
+
+namespace ecg_host {
+
+using State = std::vector<std::vector<double>>;
+
+// One printed variable of every cell (Phase/Mixture::returnScalar and
+// returnVector, src/Models/*/Phase*.cpp, Mix*.cpp).
+struct PrintedVariable {
+  std::string name;        // returnNameScalar / returnNameVector
+  int owner = 0;           // phase index, or -1 for the mixture
+  bool vector = false;     // three components per cell (x, y, z)
+  std::vector<double> v;   // ncell values, or 3 * ncell for vectors
+};
+
+// The reference's printed variables, in print order: per phase its scalars
+// then its vectors, then the mixture's scalars and vectors (multiphase only).
+std::vector<PrintedVariable> printedVariables(const CaseDescription& c, const State& s);
+// Mixture density of every cell (Euler: the phase density), as the gradient sees it.
+std::vector<double> printedDensity(const CaseDescription& c, const State& s);
+// |grad rho| of GradientFiniteDifference::computeGradients (src/Gradients/GradientFiniteDifference.cpp:45-122).
+std::vector<double> densityGradientNorm(const CaseDescription& c, const std::vector<double>& rho);
+// Persistent state rebuilt from printed variables as a restart does
+// (OutputXML::ReadPhysicalDataXML, Cell::fulfillStateRestart, Cell::completeFulfillState).
+State stateFromPrinted(const CaseDescription& c, const std::vector<PrintedVariable>& pv);
+
+// Output: the common part of the reference's Output class (src/InputOutput/Output.cpp).
+class Output {
+ public:
+  Output(const CaseDescription& c, const std::string& workFolder);
+  virtual ~Output() = default;
+  static std::unique_ptr<Output> create(const CaseDescription& c, const std::string& workFolder);
+
+  void copyInputFiles() const;                      // savesInput/ (Output.cpp:163-174)
+  void initializeOutput();                          // per-format set-up and global quantities
+  void initializeOutputInfos();                     // infoCalcul.out, infoMesh (Output.cpp:224-241)
+  void writeInfos(long long iteration, double t, double dtNext, double seconds);  // Output.cpp:494-519
+  void saveInfoCells(long long ncell);              // Output.cpp:361-372
+  void writeGlobalQuantities(double t, const State& s);   // OutputGlobalGNU.cpp:90-117
+  void prepareErrorFiles() const;                   // Errors.cpp:82-99
+  virtual void writeResults(const State& s) = 0;    // one dataset; numbers it and advances the count
+  // Restart: infoCalcul.out up to file `number` (Output::readInfos); returns the
+  // iteration, time and dt of that line and truncates the file after it.
+  void readInfos(int number, long long& iteration, double& t, double& dt);
+  virtual State readResults();                      // XML only
+
+  const std::string& folderOutput() const { return m_folderOutput; }
+  int fileNumber() const { return m_numFichier; }
+
+ protected:
+  std::string createFilename(const std::string& name, int proc, int number) const;
+  void writeDataset(std::ofstream& f, const std::vector<double>& v) const;
+  void setPrecision(std::ofstream& f) const;
+
+  const CaseDescription& m_case;
+  std::string m_folderOutput, m_folderDatasets, m_folderInfoMesh, m_folderGlobal;
+  int m_numFichier = 0;
+  int m_dim = 1;
+};
+
+class OutputGNU : public Output {
+ public:
+  using Output::Output;
+  void writeResults(const State& s) override;
+
+ private:
+  void writeScriptGnuplot(const std::vector<PrintedVariable>& pv) const;
+};
+
+class OutputXML : public Output {
+ public:
+  using Output::Output;
+  void initializeCollections() const;
+  void writeResults(const State& s) override;
+  State readResults() override;
+
+ private:
+  std::string datasetName(int number) const;
+  void writeCollections() const;
+};
+
+// A global quantity file written at every print (OutputGlobalGNU).
+void initializeGlobalQuantity(const CaseDescription& c, const std::string& folder, const std::string& quantity,
+                              bool restarted);
+double globalQuantity(const CaseDescription& c, const State& s, const std::string& quantity);
+
+// Base64 of a byte string as the reference writes it (IO::writeb64Chaine, src/InputOutput/IO.cpp:44-88).
+std::string base64(const unsigned char* data, size_t n);
+
+}  // namespace ecg_host

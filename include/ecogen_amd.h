@@ -134,6 +134,25 @@ int ecg_device_count(void);
 int ecg_read_case(const char* case_dir, const char* eos_dir, int precision, ecg_config* cfg, ecg_domain* doms,
                   int max_doms, int* ndoms, ecg_run_control* rc);
 
+/* Result files in the reference's formats (src/InputOutput/Output*.cpp) for the
+ * case in case_dir, under work_dir/results/<run>/: opening does what
+ * Run::initialize does before the first print (folders, savesInput copies,
+ * ParaView/VisIt collections, global-quantity files, and unless the case
+ * restarts, empty infoCalcul.out and infoMesh files).  Host-side only; needs no GPU. */
+typedef struct ecg_output ecg_output;
+ecg_output* ecg_output_open(const char* case_dir, const char* eos_dir, const char* work_dir);
+/* One print of Run::solver (Run.cpp:441-467) from a host state in the layout above:
+ * infoCalcul.out line, infoMesh line, global quantities, then the dataset (GNU
+ * columns + gnuplot script, or VTK XML .vtr + collections) numbered 0, 1, ... */
+int ecg_output_write(ecg_output* o, const double* const* soa, int nvars, int64_t iteration, double t,
+                     double dt_next, double seconds);
+/* Restart point of the case's <restartSimulation restartFileNumber="n"> (Run::restartSimulation,
+ * Run.cpp:345-381): the infoCalcul.out line of file n (iteration, time, dt; the file is cut
+ * after it) and the XML dataset n rebuilt into a state (Cell::fulfillStateRestart +
+ * completeFulfillState).  Later writes continue with file n + 1. */
+int ecg_output_restart(ecg_output* o, double* const* soa, int nvars, int64_t* iteration, double* t, double* dt);
+void ecg_output_close(ecg_output* o);
+
 /* Context lifetime (Run::Run / Run::initialize / Run::finalize). */
 ecg_ctx* ecg_create(const ecg_config* cfg);
 void ecg_destroy(ecg_ctx* ctx);
@@ -156,6 +175,14 @@ int ecg_step(ecg_ctx* ctx, double dt, double* dtmax_out, ecg_step_stats* st);
  * dt_{n+1} = cfl*dtMax_n kept on the device.  Stops after max_iter steps when
  * max_iter > 0, else at the first step with t >= t_end.  Restarts the clock. */
 int ecg_run(ecg_ctx* ctx, int64_t max_iter, double t_end, ecg_step_stats* st);
+/* Continues the device clock (time, dt, iteration) of the last run instead of
+ * starting afresh: runs until the first step with iteration >= until_iter
+ * (until_iter > 0) or t >= until_t.  The runner's output schedule
+ * (Run::solver, src/Run.cpp:385-498) is built from these calls. */
+int ecg_run_to(ecg_ctx* ctx, int64_t until_iter, double until_t, ecg_step_stats* st);
+/* Sets the clock a restart resumes from (Output::readInfos, src/InputOutput/Output.cpp:525-573):
+ * physical time, the dt of the next step and the iteration count. */
+int ecg_set_time(ecg_ctx* ctx, double t, double dt, int64_t iteration);
 
 /* Enqueue n steps of the device-clocked loop without synchronising (benchmarking). */
 int ecg_enqueue_steps(ecg_ctx* ctx, int n);

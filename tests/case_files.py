@@ -90,7 +90,7 @@ def write_sod(case_dir, n=200, iterations=None, t_end=5.e-4):
 """)
 
 
-def write_kapila_bubble(case_dir, n=48, iterations=12, second_order=True, model="PressureVelocityEq"):
+def write_kapila_bubble(case_dir, n=48, iterations=12, second_order=True, model="PressureVelocityEq", nz=1):
     """2D water/air: an air disc and a water rectangle in air-dominated water, a
     shocked half-space, wall on Y-, temperature-given density for the shocked air."""
     os.makedirs(case_dir, exist_ok=True)
@@ -115,8 +115,8 @@ def write_kapila_bubble(case_dir, n=48, iterations=12, second_order=True, model=
 <mesh>
   <type structure="cartesian"/>
   <cartesianMesh>
-    <dimensions x="1." y="0.75" z="1."/>
-    <numberCells x="{n}" y="{3 * n // 4}" z="1"/>
+    <dimensions x="1." y="0.75" z="{0.5 if nz > 1 else 1.}"/>
+    <numberCells x="{n}" y="{3 * n // 4}" z="{nz}"/>
     <AMR lvlMax="0" criteriaVar="0.1" varRho="true" varP="true" varU="false" varAlpha="true" xiSplit="0.1" xiJoin="0.1"/>
   </cartesianMesh>
 </mesh>
@@ -179,3 +179,21 @@ def write_kapila_bubble(case_dir, n=48, iterations=12, second_order=True, model=
   </boundaryConditions>
 {state("water", 1 - 1e-6, 1000.0, 1.e5, 0.0)}{state("air", 1e-6, 1000.0, 1.e5, 0.0)}{state("shocked", 1 - 1e-6, 1100.0, 3.e8, 120.0, air_temp=400.0)}</CI>
 """)
+
+
+def set_output(case_dir, fmt="XML", binary=False, precision=None, reduced=None, global_quantities=(),
+               restart=None):
+    """Rewrite main.xml's <outputMode> (and <globalQuantity>, <restartSimulation>)."""
+    import re
+    p = os.path.join(case_dir, "main.xml")
+    s = open(p).read()
+    attrs = f'format="{fmt}" binary="{"true" if binary else "false"}"'
+    if precision is not None:
+        attrs += f' precision="{precision}"'
+    if reduced is not None:
+        attrs += f' reducedOutput="{"true" if reduced else "false"}"'
+    extra = "".join(f'\n  <globalQuantity quantity="{q}"/>' for q in global_quantities)
+    if restart is not None:
+        extra += f'\n  <restartSimulation restartFileNumber="{restart}" AMRsaveFreq="0"/>'
+    s = re.sub(r"<outputMode[^>]*/>", f"<outputMode {attrs}/>{extra}", s)
+    _write(p, s)

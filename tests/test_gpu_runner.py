@@ -1,16 +1,20 @@
 """The command-line runner (ecogen_run) on cases written in the reference's XML
 input format: its final state must match the oracle run of the case the reader
-produces (needs a GPU)."""
+produces, its result files must match the result-file oracle fed with the
+oracle's states at the same print times, and a restart from one of its XML
+files must continue as the oracle does from the same file (needs a GPU)."""
 import os
+import re
 import subprocess
 
 import numpy as np
 import pytest
 
-from case_files import write_eos, write_kapila_bubble, write_sod
+from case_files import set_output, write_eos, write_kapila_bubble, write_sod
 
 RUNNER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ecogen_amd", "ecogen_run")
 TOL = 1e-12
+KAPILA_EOS = ["SG_water.xml", "IG_air.xml"]
 
 
 def _read_out(path):
@@ -22,35 +26,155 @@ def _read_out(path):
     return data, t, it
 
 
+def _run(case_dir, work, out):
+    r = subprocess.run([RUNNER, case_dir, "--work", work, "--out", out], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    return _read_out(out)
+
+
+def _oracle_prints(case, o, t0=0.0, it0=0, dt0=1.e-15):
+    """Run the oracle from (t0, it0, dt0) with Run::solver's print and stop rules;
+    returns [(state, iteration, t, dt_next)] of every print."""
+    ctl = case["control"]
+    prints = []
+    t, it, dt = t0, it0, dt0
+    nxt = t + ctl["time_freq"]
+    while True:
+        dtmax = o.step(dt)
+        t += dt
+        it += 1
+        dtn = case["cfl"] * dtmax
+        pr = done = False
+        if ctl["iterations_mode"]:
+            pr = it % ctl["iter_freq"] == 0
+            done = it >= ctl["iterations"]
+        else:
+            if t >= nxt:
+                pr = True
+                nxt += ctl["time_freq"]
+            if t >= ctl["total_time"]:
+                pr = done = True
+        if pr:
+            prints.append((o.get_state(), it, t, dtn))
+        dt = dtn
+        if done:
+            return prints
+
+
+_NUM = re.compile(r"^[-+]?(\d+\.?\d*|\.\d+)([eE][-+]?\d+)?$")
+
+
+def _same_text(a, b, rel):
+    """Token-wise equality; numbers may differ in the last printed digit."""
+    ta, tb = a.split(), b.split()
+    if len(ta) != len(tb):
+        return False
+    for x, y in zip(ta, tb):
+        if x == y:
+            continue
+        if not (_NUM.match(x) and _NUM.match(y)):
+            return False
+        fx, fy = float(x), float(y)
+        if abs(fx - fy) > rel * max(abs(fx), abs(fy)):
+            return False
+    return True
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("which", ["sod_time", "pueq_o2", "ueq_o1_wall"])
 def test_runner_matches_oracle(tmp_path, which):
     from ecogen_amd import read_case
+    from oracle import writers as W
     from oracle.pyoracle import Oracle
     print(f"[runner] {which}", flush=True)
     write_eos(str(tmp_path))
     d = os.path.join(str(tmp_path), which)
     if which == "sod_time":
         write_sod(d, n=300, t_end=2.e-4)
+        opts = dict(fmt="XML", precision=10, global_quantities=("mass", "totalEnergy"))
+        names = ["IG_air.xml"]
     elif which == "pueq_o2":
         write_kapila_bubble(d, n=40, iterations=15, second_order=True)
+        opts = dict(fmt="GNU", precision=10)
+        names = KAPILA_EOS
     else:
         write_kapila_bubble(d, n=40, iterations=15, second_order=False, model="VelocityEq")
-    out = os.path.join(str(tmp_path), "state.bin")
-    r = subprocess.run([RUNNER, d, "--out", out], capture_output=True, text=True, timeout=120)
-    print(r.stdout, r.stderr, flush=True)
-    assert r.returncode == 0
-    got, t, it = _read_out(out)
+        opts = dict(fmt="XML", binary=True, global_quantities=("mass",))
+        names = KAPILA_EOS
+    set_output(d, **opts)
+    work = str(tmp_path / "work")
+    os.makedirs(work)
+    got, t, it = _run(d, work, os.path.join(str(tmp_path), "state.bin"))
     case = read_case(d)
     o = Oracle(case)
-    ctl = case["control"]
-    if ctl["iterations_mode"]:
-        t_o, _, it_o = o.run(max_iter=ctl["iterations"])
-    else:
-        t_o, _, it_o = o.run(t_end=ctl["total_time"])
-    assert it == it_o and abs(t - t_o) <= 1e-12 * t_o
+    w = W.Writer(case, "", opts["fmt"], binary=opts.get("binary", False), precision=opts.get("precision", 0),
+                 eos_names=names, global_quantities=[q.lower() for q in opts.get("global_quantities", ())])
+    w.open()
+    w.write(o.get_state(), 0, 0.0, 0.0)
+    prints = _oracle_prints(case, o)
+    for st, pit, pt, pdt in prints:
+        w.write(st, pit, pt, pdt)
     ref = o.get_state()
+    assert it == prints[-1][1] and abs(t - prints[-1][2]) <= 1e-12 * t
     for v in range(ref.shape[0]):
         den = max(float(np.sum(np.abs(ref[v]))), 1e-300)
         err = float(np.sum(np.abs(got[v] - ref[v]))) / den
         assert err <= TOL or float(np.sum(np.abs(got[v] - ref[v]))) <= 1e-300, (v, err)
+    folder = os.path.join(work, "results", case["name"])
+    for rel, text in w.files.items():
+        p = os.path.join(folder, rel)
+        assert os.path.exists(p), rel
+        body = open(p).read()
+        if rel == "infoCalcul.out":
+            body = "\n".join(" ".join(l.split()[:4]) for l in body.splitlines())
+            text = "\n".join(" ".join(l.split()[:4]) for l in text.splitlines())
+        if "B64" in rel:
+            assert len(body) == len(text), rel     # base64 of doubles that may differ in the last bits
+            continue
+        assert _same_text(body, text, 1e-8), rel
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", ["PressureVelocityEq", "VelocityEq"])
+def test_runner_restart_matches_oracle(tmp_path, model):
+    from ecogen_amd import read_case
+    from oracle import writers as W
+    from oracle.pyoracle import Oracle
+    print(f"[runner] restart {model}", flush=True)
+    write_eos(str(tmp_path))
+    d = os.path.join(str(tmp_path), "b")
+    write_kapila_bubble(d, n=32, iterations=12, second_order=True, model=model)
+    s = open(os.path.join(d, "main.xml")).read().replace('iterFreq="12"', 'iterFreq="4"')
+    open(os.path.join(d, "main.xml"), "w").write(s)
+    set_output(d, fmt="XML", precision=17)
+    work = str(tmp_path / "work")
+    os.makedirs(work)
+    full, t_full, it_full = _run(d, work, os.path.join(str(tmp_path), "full.bin"))
+    assert it_full == 12
+    case = read_case(d)
+    folder = os.path.join(work, "results", case["name"])
+    info = open(os.path.join(folder, "infoCalcul.out")).read().splitlines()
+    line = info[2].split()                        # file 1: iteration 4
+    it1, t1, dt1 = int(line[1]), float(line[2]), float(line[3])
+    assert it1 == 4
+    set_output(d, fmt="XML", precision=17, restart=1)
+    got, t, it = _run(d, work, os.path.join(str(tmp_path), "restart.bin"))
+    assert it == 12
+    # oracle: the same file, the same clock
+    st = W.read_restart(case, os.path.join(folder, "datasets", "result_CPU0_TIME1.vtr"))
+    o = Oracle(case)
+    o.set_state(st)
+    prints = _oracle_prints(case, o, t0=t1, it0=it1, dt0=dt1)
+    ref = o.get_state()
+    assert [p[1] for p in prints] == [8, 12] and abs(prints[-1][2] - t) <= 1e-12 * t
+    for v in range(ref.shape[0]):
+        den = max(float(np.sum(np.abs(ref[v]))), 1e-300)
+        err = float(np.sum(np.abs(got[v] - ref[v]))) / den
+        assert err <= TOL, (v, err)
+    # the restarted run keeps the uninterrupted run's files 0-1 and rewrites 2-3
+    info2 = open(os.path.join(folder, "infoCalcul.out")).read().splitlines()
+    assert len(info2) == 5 and info2[:3] == info[:3]
+    # the restart rebuilds E from the printed variables: close to the uninterrupted run, not identical
+    rel = np.sum(np.abs(got - full), axis=1) / np.maximum(np.sum(np.abs(full), axis=1), 1e-300)
+    assert np.all(rel < 1e-9), rel
