@@ -281,8 +281,6 @@ static int check_config(const ecg_config* c) {
         return fail(ECG_E_UNSUPPORTED, "second order needs minmod/vanleer/vanalbada/superbee/mc limiters");
   for (int i = 0; i < 6; i++) {
     if (c->bc[i] != ECG_BC_NONREFLECTING && c->bc[i] != ECG_BC_WALL) return fail(ECG_E_ARG, "unknown boundary");
-    if (c->order == 2 && c->bc[i] == ECG_BC_WALL)
-      return fail(ECG_E_UNSUPPORTED, "second-order wall/symmetry boundaries (BoundCondWallO2Cartesian) are not built");
   }
   if (!(c->cfl > 0)) return fail(ECG_E_ARG, "CFL must be positive");
   if (c->precision != ECG_FP64 && c->precision != ECG_FP32_RELAX64 && c->precision != ECG_FP32_LEARNED)
@@ -347,6 +345,15 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   tile_extent(c->D, c->esize, tex);
   for (int a = 0; a < 3; a++) P.tiles[a] = (n3[a] + tex[a] - 1) / tex[a];
 
+  // Boundary faces at 0 and at the last centre + d/2 (MeshCartesian.cpp:395-552):
+  // Face::distance to the adjacent centre, used by second-order walls.
+  for (int a = 0; a < 3; a++) {
+    const double x0 = 0. - pos[a][0];
+    const double fp = pos[a][n3[a] - 1] + 0.5 * P.d[a];
+    const double x1 = fp - pos[a][n3[a] - 1];
+    P.dwall[a][0] = std::sqrt(x0 * x0 + 0. * 0. + 0. * 0.);
+    P.dwall[a][1] = std::sqrt(x1 * x1 + 0. * 0. + 0. * 0.);
+  }
   // Distance tables of the interior faces (Element.cpp:77-81, Face.cpp:52-56).
   std::vector<double> tab;
   size_t off[3][3];

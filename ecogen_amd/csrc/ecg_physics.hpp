@@ -59,6 +59,7 @@ struct Params {
   const double* dcc[3];     // cell-to-cell distance of face m (cells m | m+1)
   const double* dfl[3];     // face m to its left cell centre
   const double* dfr[3];     // face m to its right cell centre
+  double dwall[3][2];       // first / last cell centre to its boundary face, per axis
 };
 
 // Device clock: lagged time step kept on the GPU (Run.cpp:385-498).

@@ -28,6 +28,12 @@ namespace ecg {
 // are 8 x 8 x 4 (fewest halo faces per cell), 3D FP32 tiles 16 x 4 x 4 (longer
 // contiguous rows for the narrower rows of floats); both measured faster for
 // their form on the 512^3 case.
+// Limiter template argument LK of the step kernel: a limiter kind >= 0 fixes every
+// limiter of the case at compile time; LK_GENERIC reads the four kinds per face;
+// LK_WALLS does the same and also builds the second-order wall faces.
+constexpr int LK_GENERIC = -1;
+constexpr int LK_WALLS = -2;
+
 template <int D, int ES> struct Tile;
 template <int ES> struct Tile<1, ES> { static constexpr int X = 256, Y = 1, Z = 1; };
 template <int ES> struct Tile<2, ES> { static constexpr int X = 16, Y = 16, Z = 1; };
@@ -180,6 +186,44 @@ __device__ __forceinline__ void extrapolate(const Params& P, Cell<N, R>& b, cons
   }
 }
 
+// BoundCondWallO2Cartesian::computeSlopes (BoundCondWallO2Cartesian.cpp:88-112):
+// every slope zero but the wall-normal velocity, u_n / (x_cell - x_face); dw is
+// that signed distance (positive on the minus side).
+template <int MODEL, int N, int D, typename R>
+__device__ __forceinline__ void wall_slope_vars(const Cell<N, R>& c, int ax, R dw, R* o) {
+  constexpr int PV = (MODEL == M_EULER) ? 2 : 3 * N;
+#pragma unroll
+  for (int v = 0; v < PV + D; v++) o[v] = R(0.);
+  if (fabs(dw) > R(1.e-8)) o[PV + ax] = fdiv(c.u[ax], dw);
+}
+
+// Second-order wall state (BoundCondWallO2Cartesian::solveRiemann,
+// BoundCondWallO2Cartesian.cpp:116-151, with CellO2Cartesian::computeLocalSlopesLimite,
+// CellO2Cartesian.cpp:173-277): slope 1 is the wall's, slope 2 the inner face's on
+// the other side of the cell; the interface switch reads only the cell's own
+// time-n volume fraction; the extrapolated phases are not checked.
+template <int MODEL, int N, int D, typename R, int LK>
+__device__ __forceinline__ void extrapolate_wall(const Params& P, Cell<N, R>& b, const R* s1, const R* s2, R aCell,
+                                                 R dist) {
+  int lg = LK, lvf = LK;
+  if (LK < 0) {
+    if ((aCell >= R(1.e-4)) && (aCell <= R(1.) - R(1.e-4))) {
+      lg = P.lim[1]; lvf = P.lim[3];
+    } else {
+      lg = P.lim[0]; lvf = P.lim[2];
+    }
+  }
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    if (MODEL != M_EULER) { b.a[k] += limit<R>(lvf, s1[v], s2[v]) * dist; v++; }
+    b.r[k] += limit<R>(lg, s1[v], s2[v]) * dist; v++;
+    b.p[k] += limit<R>(lg, s1[v], s2[v]) * dist; v++;
+  }
+#pragma unroll
+  for (int d = 0; d < D; d++) b.u[d] = b.u[d] + limit<R>(lg, s1[v + d], s2[v + d]) * dist;
+}
+
 template <int MODEL, int N, int D, typename R>
 __device__ __forceinline__ void flux_to_lds(const Flux<N, R>& F, R* o) {
   int v = 0;
@@ -251,6 +295,9 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
   R* s_flux = lds + (O2 ? (T + 3) * LB * SZ::NS : 0);
   const int tid = threadIdx.x;
   const int lown = (own[B] - org[B]) + TB * (own[C] - org[C]);   // line of the own cell
+  // Second-order walls: their slopes enter the neighbouring cells' limiters.
+  const bool wall_m = O2 && LK == LK_WALLS && P.bc[2 * AX] == BC_WALL;
+  const bool wall_p = O2 && LK == LK_WALLS && P.bc[2 * AX + 1] == BC_WALL;
 
 #pragma unroll 1
   for (int bt = 0; bt < NB; bt++) {
@@ -269,6 +316,12 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
           load_prim<MODEL, N, D>(P, L.wsrc, cell, cl);
           load_prim<MODEL, N, D>(P, L.wsrc, cell + sa, cr);
           slope_vars<MODEL, N, D, R>(cl, cr, R(P.dcc[AX][m]), o);
+        } else if (cb < n3[B] && cc < n3[C] && ((m == -1 && wall_m) || (m == na - 1 && wall_p))) {
+          // Wall faces carry slopes (second order); the cell is the one inside.
+          const int64_t cell = (int64_t)(m < 0 ? 0 : na - 1) * sa + (int64_t)cb * st3[B] + (int64_t)cc * st3[C];
+          Cell<N, R> cw;
+          load_prim<MODEL, N, D>(P, L.wsrc, cell, cw);
+          wall_slope_vars<MODEL, N, D, R>(cw, AX, m < 0 ? R(P.dwall[AX][0]) : R(-P.dwall[AX][1]), o);
         }
       }
       __syncthreads();
@@ -290,10 +343,13 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
           const R aL = (MODEL == M_EULER) ? R(0.) : bl.a[0];
           const R aR = (MODEL == M_EULER) ? R(0.) : br.a[0];
           const R* s1 = s_slope + (lb * (T + 3) + f + 2) * SZ::NS;
-          const bool hasm = (m >= 1), hasp = (m + 1 <= na - 2);
-          // The far neighbours' time-n alpha only decides the interface limiter switch.
-          const R aLL = (LK < 0 && MODEL != M_EULER && hasm) ? alpha0_at<R>(L.wn, cell - sa) : R(0.);
-          const R aRR = (LK < 0 && MODEL != M_EULER && hasp) ? alpha0_at<R>(L.wn, cell + 2 * sa) : R(0.);
+          const bool hasm = (m >= 1) || wall_m, hasp = (m + 1 <= na - 2) || wall_p;
+          // The far neighbours' time-n alpha only decides the interface limiter switch;
+          // across a wall the "cell after" is the cell itself (CellO2Cartesian.cpp:75-79).
+          const R aLL = (LK < 0 && MODEL != M_EULER && hasm) ? alpha0_at<R>(L.wn, (m >= 1) ? cell - sa : cell) : R(0.);
+          const R aRR = (LK < 0 && MODEL != M_EULER && hasp)
+                            ? alpha0_at<R>(L.wn, (m + 1 <= na - 2) ? cell + 2 * sa : cell + sa)
+                            : R(0.);
           extrapolate<MODEL, N, D, R, LK>(P, bl, s1, s1 - SZ::NS, hasm, aLL, aL, aR, R(P.dfl[AX][m]), false);
           extrapolate<MODEL, N, D, R, LK>(P, br, s1, s1 + SZ::NS, hasp, aRR, aR, aL, R(P.dfr[AX][m]), true);
           derive<MODEL, N, R>(P, bl, MODEL == M_EULER);
@@ -335,6 +391,35 @@ __device__ __forceinline__ void boundary_face(const Params& P, const Cell<N, R>&
   const R EB = (MODEL == M_EULER) ? b.e[0] + R(0.5) * sqn3<R>(b.u) : b.E;
   boundary_flux<MODEL, N, FR, R>(P, b, EB, bc, dtm, F);
   sub_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
+}
+
+// Boundary face of axis AX, side SIDE (0: minus, 1: plus) of the own cell c.  A
+// second-order wall first extrapolates the cell's stage state to the face.
+template <int MODEL, int N, int D, bool O2, int AX, int SIDE, int FR, typename R, int LK>
+__device__ __forceinline__ void boundary_face_at(const Params& P, const StepLaunchT<R>& L, const Cell<N, R>& src,
+                                                 int64_t c, const R* an, const R* pn, R& dtm, ConsAcc<N, R>& acc) {
+  const int bc = P.bc[2 * AX + SIDE];
+  const R coef = R(P.coef[AX]);
+  if (!O2 || LK != LK_WALLS || bc != BC_WALL) {
+    boundary_face<MODEL, N, D, FR, R>(P, src, bc, coef, an, pn, dtm, acc);
+    return;
+  }
+  using SZ = Sizes<MODEL, N, D>;
+  const int n3[3] = {P.nx, P.ny, P.nz};
+  const int64_t sa = (AX == 0) ? 1 : (AX == 1) ? (int64_t)P.nx : (int64_t)P.nx * P.ny;
+  const int na = n3[AX];
+  R s1[SZ::NS], s2[SZ::NS];
+  wall_slope_vars<MODEL, N, D, R>(src, AX, SIDE == 0 ? R(P.dwall[AX][0]) : R(-P.dwall[AX][1]), s1);
+  // Slope of the inner face on the other side of the cell.
+  Cell<N, R> nb;
+  load_prim<MODEL, N, D>(P, L.wsrc, SIDE == 0 ? c + sa : c - sa, nb);
+  if (SIDE == 0) slope_vars<MODEL, N, D, R>(src, nb, R(P.dcc[AX][0]), s2);
+  else slope_vars<MODEL, N, D, R>(nb, src, R(P.dcc[AX][na - 2]), s2);
+  const R aCell = (MODEL == M_EULER) ? R(0.) : alpha0_at<R>(L.wn, c);
+  Cell<N, R> b = src;
+  extrapolate_wall<MODEL, N, D, R, LK>(P, b, s1, s2, aCell, SIDE == 0 ? R(-P.dwall[AX][0]) : R(P.dwall[AX][1]));
+  derive<MODEL, N, R>(P, b, MODEL == M_EULER);
+  boundary_face<MODEL, N, D, FR, R>(P, b, bc, coef, an, pn, dtm, acc);
 }
 
 __device__ __forceinline__ double wave_min(double v) {
@@ -431,15 +516,15 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
       Cell<N, R> src;
       load_prim<MODEL, N, D>(P, (O2 && L.stage == 2) ? L.wsrc : L.wn, c, src);
       derive<MODEL, N, R>(P, src, false);
-      if (i == 0) boundary_face<MODEL, N, D, FR_XM, R>(P, src, P.bc[0], R(P.coef[0]), an, pn, dtm, acc);
-      if (i == P.nx - 1) boundary_face<MODEL, N, D, FR_XP, R>(P, src, P.bc[1], R(P.coef[0]), an, pn, dtm, acc);
+      if (i == 0) boundary_face_at<MODEL, N, D, O2, 0, 0, FR_XM, R, LK>(P, L, src, c, an, pn, dtm, acc);
+      if (i == P.nx - 1) boundary_face_at<MODEL, N, D, O2, 0, 1, FR_XP, R, LK>(P, L, src, c, an, pn, dtm, acc);
       if (D >= 2) {
-        if (j == 0) boundary_face<MODEL, N, D, FR_YM, R>(P, src, P.bc[2], R(P.coef[1]), an, pn, dtm, acc);
-        if (j == P.ny - 1) boundary_face<MODEL, N, D, FR_YP, R>(P, src, P.bc[3], R(P.coef[1]), an, pn, dtm, acc);
+        if (j == 0) boundary_face_at<MODEL, N, D, O2, 1, 0, FR_YM, R, LK>(P, L, src, c, an, pn, dtm, acc);
+        if (j == P.ny - 1) boundary_face_at<MODEL, N, D, O2, 1, 1, FR_YP, R, LK>(P, L, src, c, an, pn, dtm, acc);
       }
       if (D >= 3) {
-        if (k == 0) boundary_face<MODEL, N, D, FR_ZM, R>(P, src, P.bc[4], R(P.coef[2]), an, pn, dtm, acc);
-        if (k == P.nz - 1) boundary_face<MODEL, N, D, FR_ZP, R>(P, src, P.bc[5], R(P.coef[2]), an, pn, dtm, acc);
+        if (k == 0) boundary_face_at<MODEL, N, D, O2, 2, 0, FR_ZM, R, LK>(P, L, src, c, an, pn, dtm, acc);
+        if (k == P.nz - 1) boundary_face_at<MODEL, N, D, O2, 2, 1, FR_ZP, R, LK>(P, L, src, c, an, pn, dtm, acc);
       }
     }
 
@@ -621,14 +706,19 @@ void launch_step_v(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counter
   hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
 }
 
-// Second order: a case whose four limiters are all minmod (the common choice)
-// runs the instantiation with the limiter fixed at compile time.
+// Second order: a case with a wall runs the instantiation that builds
+// second-order walls (kept apart: their code costs the other kernels registers);
+// otherwise a case whose four limiters are all minmod (the common choice) runs
+// the instantiation with the limiter fixed at compile time.
 template <int MODEL, int N, int D, typename R, bool LEARN>
 void launch_step_o2(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counters* cnt, hipStream_t s) {
+  bool walls = false;
+  for (int a = 0; a < D; a++) walls = walls || P.bc[2 * a] == BC_WALL || P.bc[2 * a + 1] == BC_WALL;
   const bool all_minmod = P.lim[0] == LIM_MINMOD && P.lim[1] == LIM_MINMOD && P.lim[2] == LIM_MINMOD &&
                           P.lim[3] == LIM_MINMOD;
-  if (all_minmod) launch_step_v<MODEL, N, D, true, R, LEARN, LIM_MINMOD>(P, L, clk, cnt, s);
-  else launch_step_v<MODEL, N, D, true, R, LEARN, -1>(P, L, clk, cnt, s);
+  if (walls) launch_step_v<MODEL, N, D, true, R, LEARN, LK_WALLS>(P, L, clk, cnt, s);
+  else if (all_minmod) launch_step_v<MODEL, N, D, true, R, LEARN, LIM_MINMOD>(P, L, clk, cnt, s);
+  else launch_step_v<MODEL, N, D, true, R, LEARN, LK_GENERIC>(P, L, clk, cnt, s);
 }
 
 template <int MODEL, int N, int D, typename R>
@@ -642,9 +732,9 @@ void launch_step_t(const Params& P, const StepLaunch& Lv, Clock* clk, Counters* 
   const bool learn = CAN_LEARN && Lv.mlp.frag != nullptr;
   if (Lv.stage == 0) {
     if constexpr (CAN_LEARN) {
-      if (learn) { launch_step_v<MODEL, N, D, false, R, true, -1>(P, L, clk, cnt, s); return; }
+      if (learn) { launch_step_v<MODEL, N, D, false, R, true, LK_GENERIC>(P, L, clk, cnt, s); return; }
     }
-    launch_step_v<MODEL, N, D, false, R, false, -1>(P, L, clk, cnt, s);
+    launch_step_v<MODEL, N, D, false, R, false, LK_GENERIC>(P, L, clk, cnt, s);
   } else {
     if constexpr (CAN_LEARN) {
       if (learn) { launch_step_o2<MODEL, N, D, R, true>(P, L, clk, cnt, s); return; }

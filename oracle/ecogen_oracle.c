@@ -969,9 +969,22 @@ int orc_set_state(orc_ctx* c, const double* in)
 /* Face sweeps                                                              */
 /* ------------------------------------------------------------------------ */
 
-/* Boundary face: BoundCond::computeFlux / solveRiemann (BoundCond.cpp:61-98) */
+static void boundary_face_o2_wall(orc_ctx* c, int axis, int side, int i, int j, int k, const cst* S, double coef,
+                                  double* dtMax);
+
+/* Boundary face: BoundCond::computeFlux / solveRiemann (BoundCond.cpp:61-98); second-order
+   walls extrapolate first (BoundCondWallO2Cartesian). */
 static void boundary_face(orc_ctx* c, long long L, int fr, int bc, const cst* src, double coef, double* dtMax)
 {
+  if (c->cfg.order == 2 && bc == ORC_BC_WALL) {
+    const orc_config* g = &c->cfg;
+    const long long nxy = (long long)g->nx * g->ny;
+    const int i = (int)(L % g->nx), j = (int)((L / g->nx) % g->ny), k = (int)(L / nxy);
+    const int axis = (fr == F_XM || fr == F_XP) ? 0 : (fr == F_YM || fr == F_YP) ? 1 : 2;
+    const int side = (fr == F_XP || fr == F_YP || fr == F_ZP) ? 1 : 0;
+    boundary_face_o2_wall(c, axis, side, i, j, k, src, coef, dtMax);
+    return;
+  }
   cst b = src[L];
   flux_t F;
   fulfill(c, &b);
@@ -1152,6 +1165,110 @@ static void negate_slope(const orc_ctx* c, slope_t* s)
   s->u[2] = s->u[2] * -1.;
 }
 
+/* Distance from the first (side 0) or last (side 1) cell centre of an axis to
+   its boundary face: Face::distance over the face position of
+   MeshCartesian::initializeGeometrieMonoCpu (0 for the minus face, centre + d/2
+   for the plus face, MeshCartesian.cpp:395-552). */
+static double wall_distance(const orc_ctx* c, int axis, int side)
+{
+  const int n = (axis == 0) ? c->cfg.nx : (axis == 1) ? c->cfg.ny : c->cfg.nz;
+  if (side == 0) {
+    const double x = 0. - c->pos[axis][0];
+    return norm3(x, 0., 0.);
+  } else {
+    const double fp = c->pos[axis][n - 1] + 0.5 * c->d[axis];
+    const double x = fp - c->pos[axis][n - 1];
+    return norm3(x, 0., 0.);
+  }
+}
+
+/* BoundCondWallO2Cartesian::computeSlopes (BoundCondWallO2Cartesian.cpp:88-112): every
+   slope zero but the velocity component normal to the wall, u_n / (x_cell - x_face). */
+static void wall_slope(const orc_ctx* c, const cst* s, int axis, int side, slope_t* out)
+{
+  double dw;
+  memset(out, 0, sizeof(*out));
+  dw = (side == 0) ? wall_distance(c, axis, 0) : -wall_distance(c, axis, 1);
+  if (fabs(dw) > 1.e-8) out->u[axis] = s->u[axis] / dw;
+}
+
+static int o2_wall(const orc_ctx* c, int axis, int side)
+{
+  return c->cfg.order == 2 && c->cfg.bc[2 * axis + side] == ORC_BC_WALL;
+}
+
+/* PhaseUEq/MixUEq/PhaseEuler::extrapolate without the phase check (the wall's extrapolation,
+   BoundCondWallO2Cartesian.cpp:121-133, calls no verifyAndCorrectPhase). */
+static void extrapolate_raw(const orc_ctx* c, cst* b, const slope_t* s, double dist)
+{
+  int k;
+  for (k = 0; k < c->n; k++) {
+    if (c->cfg.model != ORC_EULER) b->a[k] += s->a[k] * dist;
+    b->r[k] += s->r[k] * dist;
+    b->p[k] += s->p[k] * dist;
+    if (c->cfg.model == ORC_EULER) {
+      b->u[0] = b->u[0] + s->u[0] * dist;
+      b->u[1] = b->u[1] + s->u[1] * dist;
+      b->u[2] = b->u[2] + s->u[2] * dist;
+    }
+  }
+  if (c->cfg.model != ORC_EULER) {
+    b->u[0] = b->u[0] + s->u[0] * dist;
+    b->u[1] = b->u[1] + s->u[1] * dist;
+    b->u[2] = b->u[2] + s->u[2] * dist;
+  }
+}
+
+/* BoundCondWallO2Cartesian::solveRiemann (BoundCondWallO2Cartesian.cpp:116-151) with
+   CellO2Cartesian::computeLocalSlopesLimite (CellO2Cartesian.cpp:173-277): slope 1 is the
+   wall's, slope 2 the inner face's on the other side of the cell, the interface switch
+   looks at the cell's own time-n volume fraction only. */
+static void boundary_face_o2_wall(orc_ctx* c, int axis, int side, int i, int j, int k, const cst* S, double coef,
+                                  double* dtMax)
+{
+  const orc_config* g = &c->cfg;
+  const int n3[3] = {g->nx, g->ny, g->nz};
+  const int fr = (axis == 0) ? (side ? F_XP : F_XM) : (axis == 1) ? (side ? F_YP : F_YM) : (side ? F_ZP : F_ZM);
+  int t[3];
+  long long L = cidx(c, i, j, k);
+  cst b = S[L];
+  slope_t s1, lim, zero;
+  const slope_t* s2;
+  int q, lg, lvf;
+  double aCell, dist;
+  flux_t F;
+  memset(&zero, 0, sizeof(zero));
+  wall_slope(c, &S[L], axis, side, &s1);
+  t[0] = i; t[1] = j; t[2] = k;
+  if (side == 1) t[axis] -= 1;
+  s2 = (n3[axis] >= 2) ? &c->slopes[face_id(c, axis, t[0], t[1], t[2])] : &zero;
+  aCell = alpha0(c, &c->W[L]);
+  if ((aCell >= 1.e-4) && (aCell <= 1. - 1.e-4)) { lg = g->lim[1]; lvf = g->lim[3]; }
+  else { lg = g->lim[0]; lvf = g->lim[2]; }
+  for (q = 0; q < c->n; q++) {
+    if (c->cfg.model == ORC_EULER) {
+      lim.r[q] = orc_limiter(lg, s1.r[q], s2->r[q]);
+      lim.p[q] = orc_limiter(lg, s1.p[q], s2->p[q]);
+    } else {
+      lim.a[q] = orc_limiter(lvf, s1.a[q], s2->a[q]);
+      lim.r[q] = orc_limiter(lg, s1.r[q], s2->r[q]);
+      lim.p[q] = orc_limiter(lg, s1.p[q], s2->p[q]);
+    }
+  }
+  lim.u[0] = orc_limiter(lg, s1.u[0], s2->u[0]);
+  lim.u[1] = orc_limiter(lg, s1.u[1], s2->u[1]);
+  lim.u[2] = orc_limiter(lg, s1.u[2], s2->u[2]);
+  dist = wall_distance(c, axis, side);
+  if (side == 0) dist = -dist;   /* the minus faces' normals point to -axis */
+  extrapolate_raw(c, &b, &lim, dist);
+  fulfill(c, &b);
+  local_proj(b.u, fr);
+  if (c->cfg.model == ORC_EULER) wall_euler(&b, c->lcfl, dtMax, &F);
+  else wall_ueq(&b, c->lcfl, dtMax, &F);
+  reverse_proj(F.f.mom, fr);
+  sub_flux(c, &c->acc[L], &F, coef, &c->W[L]);
+}
+
 /* CellInterfaceO2Cartesian::solveRiemann (CellInterfaceO2Cartesian.cpp:101-202) + CellInterfaceO2::computeFlux */
 static void interior_face_o2(orc_ctx* c, int axis, int i, int j, int k, const cst* S, double* dtMax)
 {
@@ -1159,7 +1276,7 @@ static void interior_face_o2(orc_ctx* c, int axis, int i, int j, int k, const cs
   int n3[3], iL[3], iR[3];
   long long L, R;
   cst bL, bR;
-  slope_t lim;
+  slope_t lim, wopp;
   const slope_t *sref, *sopp;
   double dL, dR, aL, aR, aLL = 0., aRR = 0., fpos[3];
   flux_t F;
@@ -1187,6 +1304,11 @@ static void interior_face_o2(orc_ctx* c, int axis, int i, int j, int k, const cs
     t[axis] -= 1;
     sopp = &c->slopes[face_id(c, axis, t[0], t[1], t[2])];
     aLL = alpha0(c, &c->W[cidx(c, t[0], t[1], t[2])]);
+  } else if (o2_wall(c, axis, 0)) {
+    /* a wall carries slopes; its "cell after" is the cell itself (CellO2Cartesian.cpp:75-79) */
+    wall_slope(c, &S[L], axis, 0, &wopp);
+    sopp = &wopp;
+    aLL = alpha0(c, &c->W[L]);
   }
   local_slopes(c, sref, sopp, aLL, aL, aR, &lim);
   extrapolate(c, &bL, &lim, dL);
@@ -1198,6 +1320,10 @@ static void interior_face_o2(orc_ctx* c, int axis, int i, int j, int k, const cs
     sopp = &c->slopes[face_id(c, axis, t[0], t[1], t[2])];
     t[axis] += 1;
     aRR = alpha0(c, &c->W[cidx(c, t[0], t[1], t[2])]);
+  } else if (o2_wall(c, axis, 1)) {
+    wall_slope(c, &S[R], axis, 1, &wopp);
+    sopp = &wopp;
+    aRR = alpha0(c, &c->W[R]);
   }
   local_slopes(c, sref, sopp, aRR, aR, aL, &lim);
   negate_slope(c, &lim);

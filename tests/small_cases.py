@@ -136,5 +136,8 @@ def all_cases():
           pueq_1d(1), pueq_1d(2), pueq_2d(1), pueq_2d(2), pueq_2d(1, alpha_null=True), wall_variant(pueq_2d(1)),
           pueq3_3d(1), pueq3_3d(2),
           ueq_2d(1), ueq_2d(2), ueq_2d(1, alpha_null=True), ueq3_3d(1), ueq2_3d(2), wall_variant(ueq3_3d(1)),
-          mixed_limiters(pueq_2d(2)), mixed_limiters(ueq2_3d(2), ("vanalbada", "minmod", "mc", "superbee"))]
+          mixed_limiters(pueq_2d(2)), mixed_limiters(ueq2_3d(2), ("vanalbada", "minmod", "mc", "superbee")),
+          # second-order walls (BoundCondWallO2Cartesian)
+          wall_variant(euler_sod(2)), wall_variant(euler_2d(2)), wall_variant(euler_3d(2)), wall_variant(pueq_2d(2)),
+          wall_variant(ueq2_3d(2)), wall_variant(pueq3_3d(2)), mixed_limiters(wall_variant(pueq_2d(2)))]
     return {c["name"]: c for c in cs}

@@ -52,7 +52,7 @@ def _case(**kw):
     (dict(model="pueq"), "2 or 3 phases"),
     (dict(eos=[dict(type="IG", gamma=0.9)]), "invalid EOS"),
     (dict(order=2, limiters=("none",) * 4), "limiters"),
-    (dict(order=2, limiters=("minmod",) * 4, bc=("wall",) * 6), "not built"),
+    (dict(precision="fp32_learned"), "relaxing multiphase model"),
 ])
 def test_invalid_configurations_are_rejected(kw, msg):
     lib = ea.library()
