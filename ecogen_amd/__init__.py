@@ -20,7 +20,7 @@ _LIB = None
 MAX_PHASES = 4
 MODELS = {"euler": 0, "pueq": 1, "ueq": 2}
 EOS_TYPES = {"IG": 0, "SG": 1}
-LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5}
+LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 BCS = {"nonreflecting": 0, "wall": 1, "symmetry": 1}
 DOMAINS = {"entire": 0, "halfspace": 1, "disc": 2, "sphere": 3, "rectangle": 4, "cuboid": 5}
 PRECISIONS = {"fp64": 0, "fp32": 1, "fp32_learned": 2}
@@ -78,6 +78,7 @@ SIGNATURES = {
     "ecg_step": (C.c_int, [C.c_void_p, C.c_double, _DP, C.POINTER(EcgStepStats)]),
     "ecg_run": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(EcgStepStats)]),
     "ecg_run_to": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(EcgStepStats)]),
+    "ecg_exp": (C.c_double, [C.c_double]),
     "ecg_set_time": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int64]),
     "ecg_output_open": (C.c_void_p, [C.c_char_p, C.c_char_p, C.c_char_p]),
     "ecg_output_write": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_double,

@@ -275,10 +275,17 @@ static int check_config(const ecg_config* c) {
   for (int k = 0; k < c->nph; k++)
     if (!valid_eos(c->eos[k])) return fail(ECG_E_ARG, "invalid EOS parameters for phase " + std::to_string(k));
   if (c->order != 1 && c->order != 2) return fail(ECG_E_ARG, "order must be 1 or 2");
-  if (c->order == 2)
-    for (int i = 0; i < 4; i++)
+  if (c->order == 2) {
+    for (int i = 0; i < 4; i++) {
+      if (c->lim[i] == ECG_LIM_THINC) {
+        if (i < 2) return fail(ECG_E_ARG, "THINC can only be a volume-fraction limiter");
+        if (c->model == ECG_EULER || c->nph != 2) return fail(ECG_E_ARG, "Limiter THINC only works for 2 phases");
+        continue;
+      }
       if (c->lim[i] < ECG_LIM_MINMOD || c->lim[i] > ECG_LIM_MC)
-        return fail(ECG_E_UNSUPPORTED, "second order needs minmod/vanleer/vanalbada/superbee/mc limiters");
+        return fail(ECG_E_UNSUPPORTED, "second order needs minmod/vanleer/vanalbada/superbee/mc/THINC limiters");
+    }
+  }
   for (int i = 0; i < 6; i++) {
     if (c->bc[i] != ECG_BC_NONREFLECTING && c->bc[i] != ECG_BC_WALL) return fail(ECG_E_ARG, "unknown boundary");
   }
@@ -340,6 +347,9 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   P.relax = g.relax_p;
   for (int i = 0; i < 6; i++) P.bc[i] = g.bc[i];
   for (int i = 0; i < 4; i++) P.lim[i] = g.lim[i];
+  P.thinc = (g.order == 2 && (g.lim[2] == ECG_LIM_THINC || g.lim[3] == ECG_LIM_THINC)) ? 1 : 0;
+  P.thinc_cosh = std::cosh(1.6);
+  P.thinc_tanh = std::tanh(1.6);
   for (int k = 0; k < g.nph; k++) P.eos[k] = make_eos(g.eos[k]);
   int tex[3];
   tile_extent(c->D, c->esize, tex);
@@ -660,6 +670,8 @@ int ecg_run(ecg_ctx* c, int64_t max_iter, double t_end, ecg_step_stats* st) {
   if (rc != ECG_OK) return rc;
   return run_clock(c, fresh_clock(1.e-15, max_iter, max_iter > 0 ? INFINITY : t_end), max_iter, st);
 }
+
+double ecg_exp(double x) { return exp_cr(x); }
 
 int ecg_run_to(ecg_ctx* c, int64_t until_iter, double until_t, ecg_step_stats* st) {
   if (!c) return fail(ECG_E_ARG, "null context");

@@ -26,7 +26,7 @@
 namespace ecg {
 
 enum { M_EULER = 0, M_PUEQ = 1, M_UEQ = 2 };
-enum { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_VANALBADA = 3, LIM_SUPERBEE = 4, LIM_MC = 5 };
+enum { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_VANALBADA = 3, LIM_SUPERBEE = 4, LIM_MC = 5, LIM_THINC = 6 };
 enum { BC_NONREFLECTING = 0, BC_WALL = 1 };
 
 struct EosDev {
@@ -60,6 +60,8 @@ struct Params {
   const double* dfl[3];     // face m to its left cell centre
   const double* dfr[3];     // face m to its right cell centre
   double dwall[3][2];       // first / last cell centre to its boundary face, per axis
+  int thinc;                // a volume-fraction limiter is THINC (two phases)
+  double thinc_cosh, thinc_tanh;   // cosh(1.6), tanh(1.6) of the host's libm
 };
 
 // Device clock: lagged time step kept on the GPU (Run.cpp:385-498).
@@ -150,7 +152,7 @@ __device__ __forceinline__ double eos_rho_pfinal(const EosDev& q, double eps, do
 template <typename R>
 __device__ __forceinline__ R limit(int kind, R s1, R s2) {
   R prod = s1 * s2, out = R(0.);
-  if (kind == LIM_MINMOD) {
+  if (kind == LIM_MINMOD || kind == LIM_THINC) {   // LimiterTHINC::limiteSlope is minmod
     if (prod > R(1e-9)) { out = dmin<R>(fabs(s1), fabs(s2)); if (s1 < R(0)) out = -out; }
   } else if (kind == LIM_VANLEER) {
     R sum = s1 + s2;
@@ -170,6 +172,75 @@ __device__ __forceinline__ R limit(int kind, R s1, R s2) {
     if (s1 < R(0.)) out = -out;
   }
   return out;
+}
+
+// Correctly rounded exp (round to nearest) for -708 < x < 709.78: x = k ln2 + r
+// with ln2 and r in double-double, exp(r) by the nested form
+// 1 + r (1 + r/2 (1 + r/3 (...))) in double-double (about 2^-100 relative), then
+// one rounding and an exact scaling by 2^k.  The THINC reconstruction uses it so
+// that its result does not depend on a math library's last-bit behaviour.
+struct DD { double hi, lo; };
+__host__ __device__ inline DD dd_fast_two_sum(double a, double b) {
+  const double s = a + b;
+  return DD{s, b - (s - a)};
+}
+__host__ __device__ inline DD dd_two_sum(double a, double b) {
+  const double s = a + b, bb = s - a;
+  return DD{s, (a - (s - bb)) + (b - bb)};
+}
+__host__ __device__ inline DD dd_mul(DD a, DD b) {
+  const double p = a.hi * b.hi;
+  double e = fma(a.hi, b.hi, -p);
+  e += a.hi * b.lo + a.lo * b.hi;
+  return dd_fast_two_sum(p, e);
+}
+__host__ __device__ inline DD dd_div_int(DD a, double n) {
+  const double q1 = a.hi / n;
+  const double rem = fma(-q1, n, a.hi);   // exact
+  const double q2 = (rem + a.lo) / n;
+  return dd_fast_two_sum(q1, q2);
+}
+__host__ __device__ inline DD dd_add_one(DD a) {
+  DD s = dd_two_sum(a.hi, 1.);
+  s.lo += a.lo;
+  return dd_fast_two_sum(s.hi, s.lo);
+}
+__host__ __device__ inline double exp_cr(double x) {
+  if (!(x == x)) return x;
+  if (x >= 709.79) return 1. / 0.;
+  if (x <= -708.) return 0.;
+  const double LN2_HI = 0x1.62e42fefa39efp-1, LN2_LO = 0x1.abc9e3b39803fp-56;
+  const double k = rint(x * 0x1.71547652b82fep0);
+  const double p_hi = k * LN2_HI;
+  const double p_lo = fma(k, LN2_HI, -p_hi);        // k LN2_HI = p_hi + p_lo exactly
+  const DD s = dd_two_sum(x, -p_hi);
+  const DD r = dd_fast_two_sum(s.hi, (s.lo - p_lo) - k * LN2_LO);
+  DD acc = DD{1., 0.};
+  for (int n = 24; n >= 1; n--) acc = dd_add_one(dd_div_int(dd_mul(r, acc), (double)n));
+  return ldexp(acc.hi, (int)k);
+}
+
+// THINC volume fraction of a cell at an interface face (CellInterfaceO2Cartesian.cpp:
+// 133-149 for the face's left cell, side 0; 165-180 for its right cell, side 1):
+// a is the cell's alpha, a_after the alpha beyond its opposite face, a_other the
+// alpha across the face.  Returns false where the interface test fails.
+__device__ __forceinline__ bool thinc_alpha(const Params& P, int side, double a, double a_after, double a_other,
+                                            double& out) {
+  const double beta = 1.6, epsI = 1.e-4;
+  if (!((a >= epsI) && (a <= 1. - epsI) && ((a_other - a) * (a - a_after) >= 1.e-8))) return false;
+  const double hi = (side == 0) ? a_other : a_after, lo = (side == 0) ? a_after : a_other;
+  const double sign = (hi - lo > 0.) ? 1. : -1.;
+  const double qmin = dmin<double>(hi, lo);
+  const double qmax = dmax<double>(hi, lo) - qmin;
+  const double C = (a - qmin + 1.e-20) / (qmax + 1.e-20);
+  const double B = exp_cr(sign * beta * (2. * C - 1.));
+  const double A = (B / P.thinc_cosh - 1.) / P.thinc_tanh;
+  double na = (side == 0) ? qmin + 0.5 * qmax * (1. + sign * (P.thinc_tanh + A) / (1. + A * P.thinc_tanh))
+                          : qmin + 0.5 * qmax * (1. + sign * A);
+  if (na < epsI) na = epsI;
+  if (na > 1. - epsI) na = 1. - epsI;
+  out = na;
+  return true;
 }
 
 // ---------------------------------------------------------------------------

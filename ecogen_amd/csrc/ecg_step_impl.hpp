@@ -30,9 +30,10 @@ namespace ecg {
 // their form on the 512^3 case.
 // Limiter template argument LK of the step kernel: a limiter kind >= 0 fixes every
 // limiter of the case at compile time; LK_GENERIC reads the four kinds per face;
-// LK_WALLS does the same and also builds the second-order wall faces.
+// LK_FULL does the same and also builds second-order walls and the THINC
+// reconstruction (their code costs the other instantiations registers).
 constexpr int LK_GENERIC = -1;
-constexpr int LK_WALLS = -2;
+constexpr int LK_FULL = -2;
 
 template <int D, int ES> struct Tile;
 template <int ES> struct Tile<1, ES> { static constexpr int X = 256, Y = 1, Z = 1; };
@@ -296,8 +297,8 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
   const int tid = threadIdx.x;
   const int lown = (own[B] - org[B]) + TB * (own[C] - org[C]);   // line of the own cell
   // Second-order walls: their slopes enter the neighbouring cells' limiters.
-  const bool wall_m = O2 && LK == LK_WALLS && P.bc[2 * AX] == BC_WALL;
-  const bool wall_p = O2 && LK == LK_WALLS && P.bc[2 * AX + 1] == BC_WALL;
+  const bool wall_m = O2 && LK == LK_FULL && P.bc[2 * AX] == BC_WALL;
+  const bool wall_p = O2 && LK == LK_FULL && P.bc[2 * AX + 1] == BC_WALL;
 
 #pragma unroll 1
   for (int bt = 0; bt < NB; bt++) {
@@ -352,6 +353,14 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
                             : R(0.);
           extrapolate<MODEL, N, D, R, LK>(P, bl, s1, s1 - SZ::NS, hasm, aLL, aL, aR, R(P.dfl[AX][m]), false);
           extrapolate<MODEL, N, D, R, LK>(P, br, s1, s1 + SZ::NS, hasp, aRR, aR, aL, R(P.dfr[AX][m]), true);
+          if constexpr (LK == LK_FULL && MODEL != M_EULER && N == 2) {
+            if (P.thinc) {
+              // THINC replaces the extrapolated volume fractions at an interface.
+              double na;
+              if (thinc_alpha(P, 0, (double)aL, (double)aLL, (double)aR, na)) { bl.a[0] = R(na); bl.a[1] = R(1. - na); }
+              if (thinc_alpha(P, 1, (double)aR, (double)aRR, (double)aL, na)) { br.a[0] = R(na); br.a[1] = R(1. - na); }
+            }
+          }
           derive<MODEL, N, R>(P, bl, MODEL == M_EULER);
           derive<MODEL, N, R>(P, br, MODEL == M_EULER);
         } else {
@@ -400,7 +409,7 @@ __device__ __forceinline__ void boundary_face_at(const Params& P, const StepLaun
                                                  int64_t c, const R* an, const R* pn, R& dtm, ConsAcc<N, R>& acc) {
   const int bc = P.bc[2 * AX + SIDE];
   const R coef = R(P.coef[AX]);
-  if (!O2 || LK != LK_WALLS || bc != BC_WALL) {
+  if (!O2 || LK != LK_FULL || bc != BC_WALL) {
     boundary_face<MODEL, N, D, FR, R>(P, src, bc, coef, an, pn, dtm, acc);
     return;
   }
@@ -706,17 +715,16 @@ void launch_step_v(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counter
   hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
 }
 
-// Second order: a case with a wall runs the instantiation that builds
-// second-order walls (kept apart: their code costs the other kernels registers);
-// otherwise a case whose four limiters are all minmod (the common choice) runs
-// the instantiation with the limiter fixed at compile time.
+// Second order: a case with a wall or a THINC limiter runs the LK_FULL
+// instantiation; otherwise a case whose four limiters are all minmod (the common
+// choice) runs the instantiation with the limiter fixed at compile time.
 template <int MODEL, int N, int D, typename R, bool LEARN>
 void launch_step_o2(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counters* cnt, hipStream_t s) {
-  bool walls = false;
-  for (int a = 0; a < D; a++) walls = walls || P.bc[2 * a] == BC_WALL || P.bc[2 * a + 1] == BC_WALL;
+  bool full = P.thinc != 0;
+  for (int a = 0; a < D; a++) full = full || P.bc[2 * a] == BC_WALL || P.bc[2 * a + 1] == BC_WALL;
   const bool all_minmod = P.lim[0] == LIM_MINMOD && P.lim[1] == LIM_MINMOD && P.lim[2] == LIM_MINMOD &&
                           P.lim[3] == LIM_MINMOD;
-  if (walls) launch_step_v<MODEL, N, D, true, R, LEARN, LK_WALLS>(P, L, clk, cnt, s);
+  if (full) launch_step_v<MODEL, N, D, true, R, LEARN, LK_FULL>(P, L, clk, cnt, s);
   else if (all_minmod) launch_step_v<MODEL, N, D, true, R, LEARN, LIM_MINMOD>(P, L, clk, cnt, s);
   else launch_step_v<MODEL, N, D, true, R, LEARN, LK_GENERIC>(P, L, clk, cnt, s);
 }

@@ -121,7 +121,7 @@ Limiter Limiter::fromName(const std::string& xmlName, bool volumeFraction) {
   else if (u == "MC") l.kind = ECG_LIM_MC;
   else if (u == "THINC") {
     if (!volumeFraction) throw InputError(ECG_E_ARG, "THINC can only be a volume-fraction limiter");
-    throw InputError(ECG_E_UNSUPPORTED, "the THINC volume-fraction limiter is not built");
+    l.kind = ECG_LIM_THINC;
   } else {
     throw InputError(ECG_E_ARG, "unknown limiter " + xmlName);
   }
@@ -445,6 +445,9 @@ CaseDescription Input::readInputXML(const std::string& caseDir, const std::strin
   ecg_config& g = c.config;
   g.model = c.model.abiModel();
   g.nph = c.model.numberPhases;
+  // Input.cpp:667-668
+  if ((c.limiters[2].kind == ECG_LIM_THINC || c.limiters[3].kind == ECG_LIM_THINC) && g.nph != 2)
+    throw InputError(ECG_E_ARG, join(caseDir, "model.xml") + ": Limiter THINC only works for 2 phases");
   if (g.nph > ECG_MAX_PHASES) throw InputError(ECG_E_UNSUPPORTED, "more than four phases");
   for (int k = 0; k < g.nph; k++) g.eos[k] = c.eos[k].toAbi();
   g.order = (c.order == "SECONDORDER") ? 2 : 1;

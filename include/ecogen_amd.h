@@ -41,8 +41,11 @@ extern "C" {
 
 enum ecg_model { ECG_EULER = 0, ECG_PUEQ = 1, ECG_UEQ = 2 };
 enum ecg_eos_type { ECG_EOS_IG = 0, ECG_EOS_SG = 1 };
+/* ECG_LIM_THINC: volume-fraction limiters only (lim[2], lim[3]), two-phase models; its slope
+ * limiter is minmod and at interfaces the extrapolated volume fractions are replaced by the THINC
+ * profile's face value (LimiterTHINC.cpp, CellInterfaceO2Cartesian.cpp:133-180). */
 enum ecg_limiter { ECG_LIM_NONE = 0, ECG_LIM_MINMOD = 1, ECG_LIM_VANLEER = 2, ECG_LIM_VANALBADA = 3,
-                   ECG_LIM_SUPERBEE = 4, ECG_LIM_MC = 5 };
+                   ECG_LIM_SUPERBEE = 4, ECG_LIM_MC = 5, ECG_LIM_THINC = 6 };
 enum ecg_bc { ECG_BC_NONREFLECTING = 0, ECG_BC_WALL = 1 };
 /* Arithmetic of the hot path:
  *   ECG_FP64          plain double precision, the parity path
@@ -133,6 +136,11 @@ int ecg_device_count(void);
  * max_doms is too small, *ndoms holds the count needed and ECG_E_ARG is returned. */
 int ecg_read_case(const char* case_dir, const char* eos_dir, int precision, ecg_config* cfg, ecg_domain* doms,
                   int max_doms, int* ndoms, ecg_run_control* rc);
+
+/* The exponential the kernels use where the reference calls exp (the THINC
+ * reconstruction, CellInterfaceO2Cartesian.cpp:139,171): correctly rounded for
+ * -708 < x < 709.78, the same code on the host and the device. */
+double ecg_exp(double x);
 
 /* Result files in the reference's formats (src/InputOutput/Output*.cpp) for the
  * case in case_dir, under work_dir/results/<run>/: opening does what

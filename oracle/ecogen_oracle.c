@@ -8,6 +8,7 @@
  * source tree.
  */
 #include <math.h>
+#include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -132,6 +133,7 @@ double orc_limiter(int kind, double s1, double s2)
   double slope = 0., produit = s1 * s2;
   switch (kind) {
     case ORC_LIM_MINMOD:
+    case ORC_LIM_THINC:   /* LimiterTHINC::limiteSlope is minmod (LimiterTHINC.cpp:43-52) */
       if (produit > 1e-9) {
         slope = omin(fabs(s1), fabs(s2));
         if (s1 < 0) slope = -slope;
@@ -1152,6 +1154,56 @@ static void extrapolate(const orc_ctx* c, cst* b, const slope_t* s, double dist)
   }
 }
 
+static int g_exact_exp = 0;
+
+void orc_set_exact_exp(int on) { g_exact_exp = on; }
+
+static double thinc_exp(double x)
+{
+  if (g_exact_exp) return (double)expq((__float128)x);
+  return exp(x);
+}
+
+int orc_thinc(int side, double a, double a_after, double a_other, double* out)
+{
+  const double beta = 1.6, epsInterface = 1.e-4;
+  double sign, qmin, qmax, C, B, A, newAlpha;
+  if (!((a >= epsInterface) && (a <= 1. - epsInterface) && ((a_other - a) * (a - a_after) >= 1.e-8))) return 0;
+  /* the left cell compares the right alpha with its far-left one, the right cell its
+     far-right alpha with the left one */
+  if (side == 0) {
+    sign = (a_other - a_after > 0.) ? 1. : -1.;
+    qmin = omin(a_other, a_after);
+    qmax = omax(a_other, a_after) - qmin;
+  } else {
+    sign = (a_after - a_other > 0.) ? 1. : -1.;
+    qmin = omin(a_after, a_other);
+    qmax = omax(a_after, a_other) - qmin;
+  }
+  C = (a - qmin + 1.e-20) / (qmax + 1.e-20);
+  B = thinc_exp(sign * beta * (2. * C - 1.));
+  A = (B / cosh(beta) - 1.) / tanh(beta);
+  if (side == 0) newAlpha = qmin + 0.5 * qmax * (1. + sign * (tanh(beta) + A) / (1. + A * tanh(beta)));
+  else newAlpha = qmin + 0.5 * qmax * (1. + sign * A);
+  if (newAlpha < epsInterface) newAlpha = epsInterface;
+  if (newAlpha > 1. - epsInterface) newAlpha = 1. - epsInterface;
+  *out = newAlpha;
+  return 1;
+}
+
+/* THINC override of the extrapolated volume fractions (2 phases) when a volume-fraction limiter is THINC */
+static void thinc_override(const orc_ctx* c, cst* b, int side, double a, double a_after, double a_other)
+{
+  double na;
+  const int* lim = c->cfg.lim;
+  if (c->cfg.model == ORC_EULER || c->n != 2) return;
+  if (lim[2] != ORC_LIM_THINC && lim[3] != ORC_LIM_THINC) return;
+  if (orc_thinc(side, a, a_after, a_other, &na)) {
+    b->a[0] = na;
+    b->a[1] = 1. - na;
+  }
+}
+
 static void negate_slope(const orc_ctx* c, slope_t* s)
 {
   int k;
@@ -1312,6 +1364,7 @@ static void interior_face_o2(orc_ctx* c, int axis, int i, int j, int k, const cs
   }
   local_slopes(c, sref, sopp, aLL, aL, aR, &lim);
   extrapolate(c, &bL, &lim, dL);
+  thinc_override(c, &bL, 0, aL, aLL, aR);
   /* right cell: opposite face is (R | R+1) */
   sopp = NULL;
   if (iR[axis] <= n3[axis] - 2) {
@@ -1328,6 +1381,7 @@ static void interior_face_o2(orc_ctx* c, int axis, int i, int j, int k, const cs
   local_slopes(c, sref, sopp, aRR, aR, aL, &lim);
   negate_slope(c, &lim);
   extrapolate(c, &bR, &lim, dR);
+  thinc_override(c, &bR, 1, aR, aRR, aL);
   fulfill(c, &bL);
   fulfill(c, &bR);
   local_proj(bL.u, fr);

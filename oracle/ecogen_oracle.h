@@ -24,7 +24,7 @@ extern "C" {
 enum { ORC_EULER = 0, ORC_PUEQ = 1, ORC_UEQ = 2 };
 enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
 enum { ORC_LIM_NONE = 0, ORC_LIM_MINMOD = 1, ORC_LIM_VANLEER = 2, ORC_LIM_VANALBADA = 3,
-       ORC_LIM_SUPERBEE = 4, ORC_LIM_MC = 5 };
+       ORC_LIM_SUPERBEE = 4, ORC_LIM_MC = 5, ORC_LIM_THINC = 6 };
 enum { ORC_BC_NONREFLECTING = 0, ORC_BC_WALL = 1 };
 enum { ORC_DOM_ENTIRE = 0, ORC_DOM_HALFSPACE = 1, ORC_DOM_DISC = 2, ORC_DOM_SPHERE = 3,
        ORC_DOM_RECTANGLE = 4, ORC_DOM_CUBOID = 5 };
@@ -92,6 +92,14 @@ double orc_eos_rho_pfinal(const orc_eos* q, double eps, double p0, double rho0, 
 double orc_eos_rho_hugoniot(const orc_eos* q, double eps, double p0, double rho0, double pf);
 double orc_eos_clamp_p(const orc_eos* q, double p);
 double orc_limiter(int kind, double s1, double s2);
+/* THINC volume fraction of a cell at the interface face (CellInterfaceO2Cartesian.cpp:133-149
+   for side 0 = the face's left cell, :165-180 for side 1 = its right cell): a = the cell's
+   alpha, a_after = the alpha beyond its opposite face, a_other = the alpha across the face.
+   Returns 0 and leaves *out unchanged where the interface test fails. */
+int orc_thinc(int side, double a, double a_after, double a_other, double* out);
+/* exp inside THINC: 0 (default) the C library's exp, as the reference calls it; 1 the
+   correctly rounded value (quad-precision expq rounded once), which the GPU kernels compute. */
+void orc_set_exact_exp(int on);
 /* Euler HLLC in the face frame: in = (rho,u,v,w,p,c,E) x2, out = mass, mom[3], energy, sM. */
 void orc_hllc_euler(const double* L, const double* R, double lcfl, double* out, double* dtmax);
 /* Multiphase (mixture) HLLC in the face frame for n phases.

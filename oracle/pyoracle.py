@@ -17,7 +17,7 @@ _LIB = None
 MAXPH = 4
 MODELS = {"euler": 0, "pueq": 1, "ueq": 2}
 EOS_TYPES = {"IG": 0, "SG": 1}
-LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5}
+LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
 BCS = {"nonreflecting": 0, "wall": 1, "symmetry": 1}
 DOMAINS = {"entire": 0, "halfspace": 1, "disc": 2, "sphere": 3, "rectangle": 4, "cuboid": 5}
 
@@ -81,6 +81,9 @@ def lib():
         L.orc_eos_clamp_p.argtypes = [C.POINTER(OrcEos), C.c_double]
         L.orc_limiter.restype = C.c_double
         L.orc_limiter.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.orc_set_exact_exp.argtypes = [C.c_int]
+        L.orc_thinc.restype = C.c_int
+        L.orc_thinc.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp]
         L.orc_hllc_euler.argtypes = [dp, dp, C.c_double, dp, dp]
         L.orc_hllc_ueq.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, dp, dp, C.c_double, dp, dp]
         L.orc_relax_cell.restype = C.c_int

@@ -130,6 +130,32 @@ def mixed_limiters(case, lims=("minmod", "superbee", "vanleer", "mc")):
     return c
 
 
+def thinc(case, lims=("minmod", "minmod", "thinc", "thinc")):
+    """THINC volume-fraction limiter(s): the extrapolated volume fractions at an
+    interface come from the THINC profile (CellInterfaceO2Cartesian.cpp:133-180).
+    A one-cell band of mixed cells (alpha = 0.3) between water and air, and the
+    whole case moving 40 m/s faster along x (a Galilean shift), make the interface
+    test hold and carry volume fraction from the first step."""
+    c = dict(case)
+    c["limiters"] = lims
+    dx = c["lx"] / c["nx"]
+    x0 = 0.5 * c["lx"] + 0.25 * dx
+    u = [40., 0., 0.]
+    shifted = []
+    for d in case["domains"]:
+        d = dict(d)
+        v = list(d.get("vel", (0., 0., 0.)))
+        v[0] += 40.
+        d["vel"] = v
+        shifted.append(d)
+    c["domains"] = shifted + [
+        dict(kind="halfspace", axis=0, origin=x0, dir=1, alpha=[0.3, 0.7], rho=[1000., 1.2], p=[1.e5, 1.e5], vel=u),
+        dict(kind="halfspace", axis=0, origin=x0 + dx, dir=1, alpha=[1.e-6, 1. - 1.e-6], rho=[1000., 1.2],
+             p=[1.e5, 1.e5], vel=u)]
+    c["name"] = case["name"] + "_thinc" + ("" if lims == ("minmod", "minmod", "thinc", "thinc") else "_mix")
+    return c
+
+
 def all_cases():
     cs = [pueq_3d_tiles(2), pueq_2d_xcd(1), pueq_2d_xcd(2), euler_sod(1), euler_sod(2), euler_2d(1), euler_2d(2), euler_2d(2, lim="superbee"), euler_3d(1),
           euler_3d(2), wall_variant(euler_2d(1)), wall_variant(euler_3d(1)),
@@ -139,5 +165,8 @@ def all_cases():
           mixed_limiters(pueq_2d(2)), mixed_limiters(ueq2_3d(2), ("vanalbada", "minmod", "mc", "superbee")),
           # second-order walls (BoundCondWallO2Cartesian)
           wall_variant(euler_sod(2)), wall_variant(euler_2d(2)), wall_variant(euler_3d(2)), wall_variant(pueq_2d(2)),
-          wall_variant(ueq2_3d(2)), wall_variant(pueq3_3d(2)), mixed_limiters(wall_variant(pueq_2d(2)))]
+          wall_variant(ueq2_3d(2)), wall_variant(pueq3_3d(2)), mixed_limiters(wall_variant(pueq_2d(2))),
+          # THINC volume-fraction reconstruction
+          thinc(pueq_1d(2)), thinc(pueq_2d(2), ("minmod", "superbee", "thinc", "thinc")),
+          thinc(ueq2_3d(2), ("vanleer", "minmod", "minmod", "thinc")), thinc(wall_variant(ueq_2d(2)))]
     return {c["name"]: c for c in cs}

@@ -130,3 +130,17 @@ def test_reference_cases_in_scope_are_read():
         n_ok += 1
         assert c["nx"] > 1 and len(c["domains"]) >= 1
     assert n_ok >= 5
+
+
+def test_thinc_volume_fraction_limiter(tmp_path):
+    write_eos(str(tmp_path))
+    d = os.path.join(str(tmp_path), "b")
+    write_kapila_bubble(d, n=16, second_order=True)
+    _edit(os.path.join(d, "main.xml"), "<globalVolumeFractionLimiter>vanleer</globalVolumeFractionLimiter>",
+          "<globalVolumeFractionLimiter>THINC</globalVolumeFractionLimiter>")
+    c = read_case(d)
+    assert c["limiters"] == ("minmod", "superbee", "thinc", "superbee")
+    _edit(os.path.join(d, "main.xml"), "<interfaceLimiter>superbee</interfaceLimiter>",
+          "<interfaceLimiter>thinc</interfaceLimiter>")
+    with pytest.raises(EcgError, match="only be a volume-fraction limiter"):
+        read_case(d)
