@@ -36,7 +36,7 @@ class EcgConfig(C.Structure):
                 ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
                 ("model", C.c_int), ("nph", C.c_int), ("eos", EcgEos * MAX_PHASES), ("order", C.c_int),
                 ("lim", C.c_int * 4), ("alpha_null", C.c_int), ("relax_p", C.c_int), ("bc", C.c_int * 6),
-                ("cfl", C.c_double), ("precision", C.c_int)]
+                ("cfl", C.c_double), ("precision", C.c_int), ("relax_rate", C.c_double)]
 
 
 class EcgDomain(C.Structure):
@@ -163,6 +163,7 @@ def make_config(case):
         c.bc[i] = BCS[b.lower()]
     c.cfl = case.get("cfl", 0.8)
     c.precision = PRECISIONS[case.get("precision", "fp64")]
+    c.relax_rate = float(case.get("relax_rate", 0.0))
     return c
 
 
@@ -221,7 +222,8 @@ def read_case(case_dir, eos_dir=None, precision="fp64"):
     return dict(name=rc.run_name.decode(), nx=cfg.nx, ny=cfg.ny, nz=cfg.nz, lx=cfg.lx, ly=cfg.ly, lz=cfg.lz,
                 model=inv(MODELS, cfg.model), eos=eos, order=cfg.order,
                 limiters=tuple(inv(LIMITERS, cfg.lim[i]) for i in range(4)), alpha_null=bool(cfg.alpha_null),
-                relax_p=bool(cfg.relax_p), bc=tuple(bcs[cfg.bc[i]] for i in range(6)), cfl=cfg.cfl,
+                relax_p=bool(cfg.relax_p), relax_rate=cfg.relax_rate, bc=tuple(bcs[cfg.bc[i]] for i in range(6)),
+                cfl=cfg.cfl,
                 precision=precision, domains=domains,
                 control=dict(iterations_mode=bool(rc.iterations_mode), iterations=rc.iterations,
                              iter_freq=rc.iter_freq, total_time=rc.total_time, time_freq=rc.time_freq))

@@ -294,6 +294,11 @@ static int check_config(const ecg_config* c) {
     return fail(ECG_E_ARG, "unknown precision form");
   if (c->precision == ECG_FP32_LEARNED && (c->model == ECG_EULER || (c->model == ECG_UEQ && !c->relax_p)))
     return fail(ECG_E_ARG, "the learned warm start needs a relaxing multiphase model");
+  if (!(c->relax_rate >= 0.) || !std::isfinite(c->relax_rate)) return fail(ECG_E_ARG, "relax_rate must be >= 0");
+  if (c->relax_rate > 0. && (c->model != ECG_UEQ || !c->relax_p))
+    return fail(ECG_E_ARG, "finite-rate pressure relaxation belongs to VELOCITYEQ with relax_p");
+  if (c->relax_rate > 0. && c->precision == ECG_FP32_LEARNED)
+    return fail(ECG_E_ARG, "the learned warm start accelerates the infinite relaxation only");
   return ECG_OK;
 }
 
@@ -345,6 +350,7 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   P.cfl = g.cfl;
   P.alpha_null = g.alpha_null;
   P.relax = g.relax_p;
+  P.relax_mu = (g.model == ECG_UEQ && g.relax_p) ? g.relax_rate : 0.;
   for (int i = 0; i < 6; i++) P.bc[i] = g.bc[i];
   for (int i = 0; i < 4; i++) P.lim[i] = g.lim[i];
   P.thinc = (g.order == 2 && (g.lim[2] == ECG_LIM_THINC || g.lim[3] == ECG_LIM_THINC)) ? 1 : 0;
@@ -784,6 +790,7 @@ int ecg_attach_samples(ecg_ctx* c, ecg_samples* smp, int stride) {
   if (stride < 1) return fail(ECG_E_ARG, "sample stride must be at least 1");
   if (c->cfg.model == ECG_EULER || !c->cfg.relax_p) return fail(ECG_E_ARG, "the case has no pressure relaxation");
   if (c->mlp.frag) return fail(ECG_E_ARG, "samples record cold solves: detach the warm start first");
+  if (c->cfg.relax_rate > 0.) return fail(ECG_E_ARG, "samples record infinite relaxations; the case relaxes at a finite rate");
   // Records written so far stay; the candidate counter restarts for this attachment.
   const long long held = samples_count(smp);
   if (held < 0) return fail(ECG_E_HIP, "reading the sample counter failed");

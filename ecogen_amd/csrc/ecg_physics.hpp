@@ -60,6 +60,7 @@ struct Params {
   const double* dfl[3];     // face m to its left cell centre
   const double* dfr[3];     // face m to its right cell centre
   double dwall[3][2];       // first / last cell centre to its boundary face, per axis
+  double relax_mu;          // > 0: finite-rate pressure relaxation with this rate (VELOCITYEQ)
   int thinc;                // a volume-fraction limiter is THINC (two phases)
   double thinc_cosh, thinc_tanh;   // cosh(1.6), tanh(1.6) of the host's libm
 };
@@ -119,6 +120,11 @@ __device__ __forceinline__ R eos_pressure(const EosDev& q, R rho, R e) {
 template <typename R>
 __device__ __forceinline__ R eos_sound(const EosDev& q, R eps, R rho, R p) {
   return fsqrt(fdiv(R(q.g) * (q.sg ? (p + R(q.pinf)) : p), dmax<R>(rho, eps)));
+}
+
+// EosSG/EosIG::computeDensityTimesInterfaceSoundSpeedSquare (EosSG.cpp:115-118, EosIG.cpp:113-116)
+__device__ __forceinline__ double eos_rho_ci2(const EosDev& q, double pI, double p) {
+  return q.sg ? q.gm1 * pI + p + q.gpinf : q.gm1 * pI + p;
 }
 template <typename R>
 __device__ __forceinline__ R eos_p_isentropic(const EosDev& q, R eps, R p0, R rho0, R rhof) {
@@ -787,6 +793,144 @@ __device__ __forceinline__ int relax_warm(const Params& P, Cell<N>& s, double gu
   }
   fellback = true;
   return it + relax_newton<N>(P, s, p_cold, fres);
+}
+
+// RelaxationPFinite::relaxation with the Euler variable-step solver
+// (RelaxationPFinite.cpp:69-275) over dt: sub-steps of d(alpha_k)/dt = mu Dp_k with
+// the phase energies driven by the interface pressure, each sub-step at most half
+// the time to the volume-fraction and pressure bounds; once the pressures agree to
+// 1e-5 relative (or after 100 sub-steps) the infinite relaxation of the entry state
+// finishes.  Returns the sub-steps plus the Newton evaluations.  FP64.
+template <int N>
+__device__ int relax_finite(const Params& P, Cell<N>& s, double dt) {
+  const double epsilon = 1.e-15, thrA = 1.e-8, crit = 1.e-5, mu = P.relax_mu;
+  if (dt < 1.e-20) return 0;
+  if (P.eps > 1.e-20) {
+#pragma unroll
+    for (int k = 0; k < N; k++)
+      if (s.a[k] > (1. - thrA)) return 0;
+  }
+  double ak[N], rk[N], pk[N], e[N], cs[N], dp[N], z[N], rc[N];
+  bool anull[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    ak[k] = s.a[k]; rk[k] = s.r[k]; pk[k] = s.p[k];
+    anull[k] = ak[k] < thrA;
+    e[k] = eos_energy<double>(P.eos[k], P.eps, s.r[k], s.p[k]);
+    cs[k] = eos_sound<double>(P.eos[k], P.eps, s.r[k], s.p[k]);
+  }
+  // Interface pressure over the phases present (RelaxationP::computeInterfacePressure).
+  auto interface_p = [&]() {
+    double pI = 0., sumZk = 0.;
+#pragma unroll
+    for (int k = 0; k < N; k++) z[k] = s.r[k] * cs[k];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      if (!anull[k]) {
+        sumZk += z[k];
+        double sumZj = 0.;
+#pragma unroll
+        for (int j = 0; j < N; j++)
+          if (!anull[j] && j != k) sumZj += z[j];
+        pI += sumZj * (s.p[k] - 0.);
+      }
+    }
+    pI /= sumZk * (double)(N - 1);
+#pragma unroll
+    for (int k = 0; k < N; k++) pI = eos_clamp_p<double>(P.eos[k], pI);
+    return pI;
+  };
+  bool relaxed = true;
+  int iter = 0, total = 0;
+  double tlocal = 0.;
+  while ((dt - tlocal) > 1.e-15) {
+    iter++;
+    double dtlocal = dt - tlocal;
+    double pMix = 0.;
+#pragma unroll
+    for (int k = 0; k < N; k++) pMix += s.a[k] * s.p[k];
+    relaxed = true;
+#pragma unroll
+    for (int k = 0; k < N; k++)
+      if (fabs((s.p[k] - pMix) / s.p[k]) > crit) relaxed = false;
+    if (relaxed) break;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      dp[k] = 0.;
+      if (!anull[k]) {
+#pragma unroll
+        for (int j = 0; j < N; j++)
+          if (!anull[j] && j != k) dp[k] += s.a[j] * (s.p[k] - 0. - s.p[j] + 0.);
+        dp[k] *= s.a[k];
+      }
+    }
+    const double pI = interface_p();
+    if (mu > 1.e-20) {
+      double dtmax = 1.e10;
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        const double sA = mu * dp[k];
+        if (sA > epsilon) dtmax = fmin(dtmax, (1. - s.a[k]) / sA);
+        else if (sA < -epsilon) dtmax = fmin(dtmax, -s.a[k] / sA);
+      }
+#pragma unroll
+      for (int k = 0; k < N; k++) rc[k] = eos_rho_ci2(P.eos[k], pI, s.p[k]);
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        if (s.a[k] > thrA) {
+          double sP = -rc[k] * dp[k] / s.a[k];
+#pragma unroll
+          for (int j = 0; j < N; j++) sP -= (s.p[j] - rc[j]) * dp[j];
+          const double dtnew = (pMix - s.p[k]) / mu / sP;
+          if (dtnew > 0.) dtmax = fmin(dtmax, dtnew);
+        }
+      }
+      dtmax *= 0.5;
+      if (dtmax < dtlocal) dtlocal = dtmax;
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      if (!anull[k]) {
+        const double a0 = s.a[k];
+        e[k] = e[k] - dtlocal * mu * pI * dp[k] / dmax<double>(a0 * s.r[k], epsilon);
+        s.a[k] = a0 + dtlocal * mu * dp[k];
+        s.r[k] = a0 * s.r[k] / dmax<double>(s.a[k], epsilon);
+        s.p[k] = eos_pressure<double>(P.eos[k], s.r[k], e[k]);
+        verify_correct<M_UEQ, N, double>(P, s, k);
+        e[k] = eos_energy<double>(P.eos[k], P.eps, s.r[k], s.p[k]);
+        cs[k] = eos_sound<double>(P.eos[k], P.eps, s.r[k], s.p[k]);
+      }
+    }
+    total++;
+    tlocal = tlocal + dtlocal;
+    if (P.eps > 1.e-20) {
+      bool alphaSmall = false;
+#pragma unroll
+      for (int k = 0; k < N; k++)
+        if (s.a[k] > (1. - thrA)) alphaSmall = true;
+      if (alphaSmall) break;
+    }
+    if (iter == 100) {
+      relaxed = true;
+      break;
+    }
+  }
+  if (relaxed && mu > 1.e-20) {
+    double pStar = 0., akS[N], rkS[N], fres;
+#pragma unroll
+    for (int k = 0; k < N; k++) pStar += ak[k] * pk[k];
+    const int it = newton_core<N>(P, ak, rk, pk, pStar, akS, rkS, 100, fres);
+    total += it;
+    if (it < 100) {
+#pragma unroll
+      for (int k = 0; k < N; k++) { s.a[k] = akS[k]; s.r[k] = rkS[k]; s.p[k] = pStar; }
+    } else {
+      const double pI = interface_p();
+#pragma unroll
+      for (int k = 0; k < N; k++) s.p[k] = pI;
+    }
+  }
+  return total;
 }
 
 template <int N>

@@ -511,6 +511,7 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
 
   Cell<N> nw;
   bool relax_now = false, spread_ok = false;
+  double relax_dt = 0.;   // dt of the stage's relaxation (dt/2 after the predictor)
   double p_cold = 0., spread = 0., sigma = 1e-9, t_lin_d = 0.;
   float feat[MLP_IN];
   float t_lin = 0.f;
@@ -557,10 +558,11 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
     // the conservative variables from the primitives, FluxUEq::buildCons).
     if (MODEL != M_EULER) mixture_density<N>(own64);
     const double scale = (O2 && L.stage == 1) ? 0.5 * dt : dt;
+    relax_dt = scale;
     Cons<N> A;
     time_evolution<MODEL, N, D, R>(P, acc, A, own64, scale);
     build_prim<MODEL, N>(P, A, nw);
-    if (MODEL != M_EULER && P.relax) {
+    if (MODEL != M_EULER && P.relax && !(MODEL == M_UEQ && P.relax_mu > 0.)) {
       relax_now = !relax_gated<N>(P, nw);
       if (relax_now && (LEARN || L.sink.buf != nullptr))
         spread_ok = relax_features<N>(P, nw, p_cold, spread, feat, t_lin, t_lin_d, sigma);
@@ -579,7 +581,9 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
     // Relaxation: after the predictor (CellO2.cpp:128-134) and after the full step
     // (Run::advancingProcedure, Run.cpp:593).
     double fres = 0.;
-    if (LEARN && spread_ok) {
+    if (MODEL == M_UEQ && P.relax_mu > 0.) {
+      it = relax_finite<N>(P, nw, relax_dt);
+    } else if (LEARN && spread_ok) {
       bool fell = false;
       const double guess = p_cold + spread * (t_lin_d + sigma * (double)y);
       it = relax_warm<N>(P, nw, guess, p_cold, WARM_MAX_IT, fell, fres);

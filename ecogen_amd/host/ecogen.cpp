@@ -299,9 +299,20 @@ void read_model(const std::string& file, const std::string& eosDir, CaseDescript
     if (!t) bad(file, "<relaxation> needs attribute type");
     if (upper(*t) != "P") unsupported(file, "relaxation " + *t + " is not built (infinite pressure relaxation is)");
     const std::string* sp = r->attr("speed");
-    if (sp && upper(*sp) == "FINITE") unsupported(file, "finite-rate pressure relaxation is not built");
     if (!c.model.relaxations.empty()) bad(file, "pressure relaxation given twice");
-    c.model.relaxations.push_back(Relaxation());
+    Relaxation rx;
+    if (sp && upper(*sp) == "FINITE") {
+      // RelaxationPFinite::RelaxationPFinite (RelaxationPFinite.cpp:48-62)
+      rx.infinite = false;
+      rx.rate = num(*r, "rate", file);
+      const std::string* so = r->attr("solver");
+      if (!so) bad(file, "<relaxation> needs attribute solver");
+      const std::string su = upper(*so);
+      if (su == "LSODA") unsupported(file, "the LSODA solver of the finite-rate pressure relaxation is not built (Euler is)");
+      if (su != "EULER") bad(file, "<relaxation> solver must be Euler or LSODA");
+      if (!(rx.rate > 0.)) unsupported(file, "a finite relaxation rate of zero is not built");
+    }
+    c.model.relaxations.push_back(rx);
   }
 }
 
@@ -454,6 +465,12 @@ CaseDescription Input::readInputXML(const std::string& caseDir, const std::strin
   for (int q = 0; q < 4; q++) g.lim[q] = c.limiters[q].kind;
   g.alpha_null = c.model.alphaNull ? 1 : 0;
   g.relax_p = (g.model == ECG_PUEQ || !c.model.relaxations.empty()) ? 1 : 0;
+  g.relax_rate = 0.;
+  if (!c.model.relaxations.empty() && !c.model.relaxations[0].infinite) {
+    if (g.model != ECG_UEQ)
+      throw InputError(ECG_E_UNSUPPORTED, join(caseDir, "model.xml") + ": finite-rate pressure relaxation is built for VelocityEq");
+    g.relax_rate = c.model.relaxations[0].rate;
+  }
   g.cfl = c.control.cfl;
   g.precision = precision;
   return c;

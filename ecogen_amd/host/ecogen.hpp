@@ -47,6 +47,7 @@ class Relaxation {
  public:
   std::string type = "P";
   bool infinite = true;
+  double rate = 0.;              // finite speed: relaxation coefficient (Euler solver)
 };
 
 // Flow model of model.xml (Input.cpp:564-913).

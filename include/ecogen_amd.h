@@ -84,6 +84,8 @@ typedef struct {
   int bc[6];              /* X-, X+, Y-, Y+, Z-, Z+ (ecg_bc) */
   double cfl;
   int precision;          /* ecg_precision */
+  double relax_rate;      /* > 0 (VELOCITYEQ with relax_p): <relaxation type="P" speed="finite" rate=...
+                             solver="Euler"> (RelaxationPFinite) instead of the infinite relaxation */
 } ecg_config;
 
 /* One initial-condition domain with its state (GeometricalDomain + Phase/Mixture states). */

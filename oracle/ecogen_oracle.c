@@ -748,9 +748,162 @@ int orc_relax_cell(int n, const orc_eos* eos, double eps, int alpha_null, double
   return relax_state(n, eos, eps, a, r, p, cs, pstar_out);
 }
 
-static void relax_cell(const orc_ctx* c, cst* s, orc_stats* st)
+/* RelaxationP::computeInterfacePressure (RelaxationP.cpp:74-93) with the alphaNull mask */
+static double interface_pressure(int n, const double* z, const double* p, const int* anull)
 {
-  int it = relax_state(c->n, c->cfg.eos, c->eps, s->a, s->r, s->p, s->c, NULL);
+  double pI = 0., sumZk = 0., sumZj;
+  int k, j;
+  for (k = 0; k < n; k++) {
+    if (!anull[k]) {
+      sumZk += z[k];
+      sumZj = 0.;
+      for (j = 0; j < n; j++)
+        if (!anull[j] && j != k) sumZj += z[j];
+      pI += sumZj * (p[k] - 0.);
+    }
+  }
+  return pI /= sumZk * (double)(n - 1);
+}
+
+/* PhaseUEq::verifyAndCorrectPhase (PhaseUEq.cpp:355-368) */
+static void phase_check(const orc_eos* q, double eps, double* a, double* r, double* p)
+{
+  if (eps > 1.e-20) {
+    if (*a < 0.) *a = 0.;
+    if (*a > 1.) *a = 1.;
+    if (*r <= 1.e-15) *r = 1.e-15;
+  } else {
+    if (*a < 1e-15) *a = 1e-14;
+    if (*a > 1. - 1e-15) *a = 1. - 1e-14;
+    if (*r < 1.e-15) *r = 1.e-14;
+  }
+  *p = orc_eos_clamp_p(q, *p);
+}
+
+/* EosSG/EosIG::computeDensityTimesInterfaceSoundSpeedSquare (EosSG.cpp:115-118, EosIG.cpp:113-116) */
+static double rho_cI2(const orc_eos* q, double pI, double p)
+{
+  if (q->type == ORC_EOS_SG) return (q->gamma - 1.) * pI + p + q->gamma * q->pinf;
+  return (q->gamma - 1.) * pI + p;
+}
+
+int orc_relax_finite(int n, const orc_eos* eos, double eps, double rate, double dt, double* a, double* r,
+                     double* p)
+{
+  const double epsilon = 1.e-15, thresholdAlpha = 1.e-8, criterion = 1.e-5;
+  const int iterMax = 100;
+  const double mu = rate;
+  double ak[MAXPH], rhok[MAXPH], pk[MAXPH], e[MAXPH], csnd[MAXPH], z[MAXPH], dp[MAXPH], rc[MAXPH];
+  int anull[MAXPH], k, j, relaxed = 1, iter = 0, alphaSmall = 0, total = 0;
+  double tlocal = 0., dtlocal, dtmax, dtnew, pI, pMix, sA, sP, a0;
+  if (dt < 1.e-20) return 0;
+  if (eps > 1.e-20)
+    for (k = 0; k < n; k++)
+      if (a[k] > (1. - thresholdAlpha)) return 0;
+  for (k = 0; k < n; k++) {
+    ak[k] = a[k];
+    rhok[k] = r[k];
+    pk[k] = p[k];
+    anull[k] = (ak[k] < thresholdAlpha);
+    e[k] = orc_eos_energy(&eos[k], eps, r[k], p[k]);   /* extendedCalculusPhase after buildPrim */
+    csnd[k] = orc_eos_sound(&eos[k], eps, r[k], p[k]);
+  }
+  while ((dt - tlocal) > 1.e-15) {
+    iter++;
+    dtlocal = dt - tlocal;
+    pMix = 0.;
+    for (k = 0; k < n; k++) pMix += a[k] * p[k];
+    relaxed = 1;
+    for (k = 0; k < n; k++)
+      if (fabs((p[k] - pMix) / p[k]) > criterion) relaxed = 0;
+    if (relaxed) break;
+    /* computePressureDifferences (RelaxationPFinite.cpp:286-303) */
+    for (k = 0; k < n; k++) {
+      dp[k] = 0.;
+      if (!anull[k]) {
+        for (j = 0; j < n; j++)
+          if (!anull[j] && j != k) dp[k] += a[j] * (p[k] - 0. - p[j] + 0.);
+        dp[k] *= a[k];
+      }
+    }
+    for (k = 0; k < n; k++) z[k] = r[k] * csnd[k];
+    pI = interface_pressure(n, z, p, anull);
+    for (k = 0; k < n; k++) pI = orc_eos_clamp_p(&eos[k], pI);
+    if (mu > 1.e-20) {
+      dtmax = 1.e10;
+      for (k = 0; k < n; k++) {
+        sA = mu * dp[k];
+        if (sA > epsilon) dtmax = fmin(dtmax, (1. - a[k]) / sA);
+        else if (sA < -epsilon) dtmax = fmin(dtmax, -a[k] / sA);
+      }
+      for (k = 0; k < n; k++) rc[k] = rho_cI2(&eos[k], pI, p[k]);
+      for (k = 0; k < n; k++) {
+        if (a[k] > thresholdAlpha) {
+          sP = -rc[k] * dp[k] / a[k];
+          for (j = 0; j < n; j++) sP -= (p[j] - rc[j]) * dp[j];
+          dtnew = (pMix - p[k]) / mu / sP;
+          if (dtnew > 0.) dtmax = fmin(dtmax, dtnew);
+        }
+      }
+      dtmax *= 0.5;
+      if (dtmax < dtlocal) dtlocal = dtmax;
+    }
+    for (k = 0; k < n; k++) {
+      if (!anull[k]) {
+        a0 = a[k];
+        e[k] = e[k] - dtlocal * mu * pI * dp[k] / omax(a0 * r[k], epsilon);
+        a[k] = a0 + dtlocal * mu * dp[k];
+        r[k] = a0 * r[k] / omax(a[k], epsilon);
+        p[k] = orc_eos_pressure(&eos[k], r[k], e[k]);
+        phase_check(&eos[k], eps, &a[k], &r[k], &p[k]);
+        e[k] = orc_eos_energy(&eos[k], eps, r[k], p[k]);
+        csnd[k] = orc_eos_sound(&eos[k], eps, r[k], p[k]);
+      }
+    }
+    total++;
+    tlocal = tlocal + dtlocal;
+    if (eps > 1.e-20) {
+      for (k = 0; k < n; k++)
+        if (a[k] > (1. - thresholdAlpha)) alphaSmall = 1;
+      if (alphaSmall) break;
+    }
+    if (iter == iterMax) {
+      relaxed = 1;
+      break;
+    }
+  }
+  if (relaxed && mu > 1.e-20) {
+    /* infinite relaxation from the state saved at entry (RelaxationPFinite.cpp:243-272) */
+    double pStar = 0., akS[MAXPH], rhokS[MAXPH];
+    int it;
+    for (k = 0; k < n; k++) pStar += ak[k] * pk[k];
+    it = newton(n, eos, eps, ak, rhok, pk, &pStar, akS, rhokS);
+    total += it;
+    if (it < 100) {
+      for (k = 0; k < n; k++) {
+        a[k] = akS[k];
+        r[k] = rhokS[k];
+        p[k] = pStar;
+      }
+    } else {
+      for (k = 0; k < n; k++) z[k] = r[k] * csnd[k];
+      pI = interface_pressure(n, z, p, anull);
+      for (k = 0; k < n; k++) pI = orc_eos_clamp_p(&eos[k], pI);
+      for (k = 0; k < n; k++) p[k] = pI;
+    }
+  }
+  return total;
+}
+
+static void relax_cell(const orc_ctx* c, cst* s, double dt, orc_stats* st)
+{
+  int it;
+  if (c->cfg.relax_rate > 0.) {
+    it = orc_relax_finite(c->n, c->cfg.eos, c->eps, c->cfg.relax_rate, dt, s->a, s->r, s->p);
+    if (st && it > 0) { st->relax_calls++; st->newton_iters += it; }
+    return;
+  }
+  it = relax_state(c->n, c->cfg.eos, c->eps, s->a, s->r, s->p, s->c, NULL);
   if (st && it > 0) {
     st->relax_calls++;
     st->newton_iters += it;
@@ -1409,10 +1562,10 @@ static void sweep_o2(orc_ctx* c, const cst* S, double* dtMax)
 }
 
 /* Run::solveRelaxations (Run.cpp:771-804) applied to one state */
-static void relax_and_correct(orc_ctx* c, cst* s, orc_stats* st)
+static void relax_and_correct(orc_ctx* c, cst* s, double dt, orc_stats* st)
 {
   if (c->cfg.relax_p) {
-    relax_cell(c, s, st);
+    relax_cell(c, s, dt, st);
     correct_and_fulfill(c, s);
   }
 }
@@ -1439,7 +1592,7 @@ int orc_step(orc_ctx* c, double dt, double* dtmax_out, orc_stats* st)
       c->Ws[i] = c->W[i];
       time_evolution(c, &c->acc[i], &c->W[i], 0.5 * dt);
       build_prim(c, &c->acc[i], &c->Ws[i]);
-      if (c->cfg.relax_p) relax_cell(c, &c->Ws[i], st);
+      if (c->cfg.relax_p) relax_cell(c, &c->Ws[i], 0.5 * dt, st);
       correct_and_fulfill(c, &c->Ws[i]);
     }
     memset(c->acc, 0, sizeof(cons_t) * (size_t)c->ncell);
@@ -1452,7 +1605,7 @@ int orc_step(orc_ctx* c, double dt, double* dtmax_out, orc_stats* st)
     }
   }
   /* Run::advancingProcedure step 4 (Run.cpp:593) */
-  for (i = 0; i < c->ncell; i++) relax_and_correct(c, &c->W[i], st);
+  for (i = 0; i < c->ncell; i++) relax_and_correct(c, &c->W[i], dt, st);
   if (dtmax_out) *dtmax_out = dtMax;
   return 0;
 }

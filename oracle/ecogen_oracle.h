@@ -46,6 +46,7 @@ typedef struct {
   int relax_p;         /* infinite pressure relaxation (forced on for PUEQ) */
   int bc[6];           /* X-, X+, Y-, Y+, Z-, Z+ */
   double cfl;
+  double relax_rate;   /* > 0: finite-rate pressure relaxation (Euler solver) with this rate instead */
 } orc_config;
 
 typedef struct {
@@ -108,6 +109,11 @@ void orc_hllc_euler(const double* L, const double* R, double lcfl, double* out, 
 void orc_hllc_ueq(int n, const orc_eos* eos, double eps, const double* L, const double* R, double lcfl,
                   double* out, double* dtmax);
 /* Infinite pressure relaxation of one cell: a/r/p updated in place; returns Newton iterations. */
+/* RelaxationPFinite::relaxation with the Euler variable-step solver (RelaxationPFinite.cpp:69-275)
+   on one cell state over dt; returns the number of Euler sub-steps (the final Newton's evaluations
+   added when the pressures relaxed). */
+int orc_relax_finite(int n, const orc_eos* eos, double eps, double rate, double dt, double* a, double* r,
+                     double* p);
 int orc_relax_cell(int n, const orc_eos* eos, double eps, int alpha_null, double* a, double* r, double* p,
                    double* pstar_out);
 

@@ -32,7 +32,7 @@ class OrcConfig(C.Structure):
                 ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
                 ("model", C.c_int), ("nph", C.c_int), ("eos", OrcEos * MAXPH), ("order", C.c_int),
                 ("lim", C.c_int * 4), ("alpha_null", C.c_int), ("relax_p", C.c_int), ("bc", C.c_int * 6),
-                ("cfl", C.c_double)]
+                ("cfl", C.c_double), ("relax_rate", C.c_double)]
 
 
 class OrcDomain(C.Structure):
@@ -86,6 +86,8 @@ def lib():
         L.orc_thinc.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp]
         L.orc_hllc_euler.argtypes = [dp, dp, C.c_double, dp, dp]
         L.orc_hllc_ueq.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, dp, dp, C.c_double, dp, dp]
+        L.orc_relax_finite.restype = C.c_int
+        L.orc_relax_finite.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, C.c_double, C.c_double, dp, dp, dp]
         L.orc_relax_cell.restype = C.c_int
         L.orc_relax_cell.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, C.c_int, dp, dp, dp, dp]
         _LIB = L
@@ -118,6 +120,7 @@ def make_config(case):
     for i, b in enumerate(case.get("bc", ("nonreflecting",) * 6)):
         c.bc[i] = BCS[b.lower()]
     c.cfl = case.get("cfl", 0.8)
+    c.relax_rate = float(case.get("relax_rate", 0.0))
     return c
 
 

@@ -156,6 +156,14 @@ def thinc(case, lims=("minmod", "minmod", "thinc", "thinc")):
     return c
 
 
+def finite_relax(case, rate):
+    """Finite-rate pressure relaxation (RelaxationPFinite, Euler solver) instead of the infinite one."""
+    c = dict(case)
+    c["relax_rate"] = rate
+    c["name"] = case["name"] + f"_fin{rate:g}"
+    return c
+
+
 def all_cases():
     cs = [pueq_3d_tiles(2), pueq_2d_xcd(1), pueq_2d_xcd(2), euler_sod(1), euler_sod(2), euler_2d(1), euler_2d(2), euler_2d(2, lim="superbee"), euler_3d(1),
           euler_3d(2), wall_variant(euler_2d(1)), wall_variant(euler_3d(1)),
@@ -168,5 +176,8 @@ def all_cases():
           wall_variant(ueq2_3d(2)), wall_variant(pueq3_3d(2)), mixed_limiters(wall_variant(pueq_2d(2))),
           # THINC volume-fraction reconstruction
           thinc(pueq_1d(2)), thinc(pueq_2d(2), ("minmod", "superbee", "thinc", "thinc")),
-          thinc(ueq2_3d(2), ("vanleer", "minmod", "minmod", "thinc")), thinc(wall_variant(ueq_2d(2)))]
+          thinc(ueq2_3d(2), ("vanleer", "minmod", "minmod", "thinc")), thinc(wall_variant(ueq_2d(2))),
+          # finite-rate pressure relaxation: slow (sub-steps over the whole dt) and fast (closing Newton)
+          finite_relax(ueq_2d(1), 1e-7), finite_relax(ueq_2d(2), 1e-4), finite_relax(ueq3_3d(1), 1.0),
+          finite_relax(ueq3_3d(2), 1e-4)]
     return {c["name"]: c for c in cs}

@@ -144,3 +144,16 @@ def test_thinc_volume_fraction_limiter(tmp_path):
           "<interfaceLimiter>thinc</interfaceLimiter>")
     with pytest.raises(EcgError, match="only be a volume-fraction limiter"):
         read_case(d)
+
+
+def test_finite_rate_relaxation(tmp_path):
+    write_eos(str(tmp_path))
+    d = os.path.join(str(tmp_path), "b")
+    write_kapila_bubble(d, n=16, second_order=False, model="VelocityEq")
+    p = os.path.join(d, "model.xml")
+    _edit(p, '<relaxation type="P" speed="infinite"/>', '<relaxation type="P" speed="finite" rate="7.5e-4" solver="euler"/>')
+    c = read_case(d)
+    assert c["relax_p"] is True and c["relax_rate"] == 7.5e-4
+    _edit(p, 'solver="euler"', 'solver="LSODA"')
+    with pytest.raises(EcgError, match="LSODA"):
+        read_case(d)

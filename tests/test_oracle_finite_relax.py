@@ -1,0 +1,129 @@
+"""Pins of the oracle's finite-rate pressure relaxation (RelaxationPFinite with
+the Euler variable-step solver, RelaxationPFinite.cpp:69-275).
+
+The relaxation integrates d(alpha_k)/dt = mu Dp_k, d(e_k)/dt = -mu p_I Dp_k /
+(alpha_k rho_k) at constant alpha_k rho_k, Dp_k = alpha_k sum_j alpha_j (p_k - p_j),
+p_I the impedance-weighted interface pressure.  Checked here: against an
+independent high-accuracy integration of that system (scipy) where one Euler
+step is accurate; exact conservation of alpha_k rho_k, of sum alpha_k and of
+the mixture internal energy sum alpha_k rho_k e_k; pressure differences that
+shrink as dt grows; and, once the pressures relax within dt, the result of the
+infinite relaxation of the initial state bit for bit (the reference finishes
+with that Newton solve)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+from oracle.pyoracle import lib, make_eos
+from ecogen_amd.case import IG_AIR, SG_EPOXY, SG_WATER
+
+DP = C.POINTER(C.c_double)
+
+
+def _eos_arr(eos):
+    arr = (type(make_eos(eos[0])) * len(eos))()
+    for k, e in enumerate(eos):
+        arr[k] = make_eos(e)
+    return arr
+
+
+def _finite(eos, a, r, p, rate, dt, eps=0.0):
+    L = lib()
+    a, r, p = (np.array(x, dtype=float) for x in (a, r, p))
+    n = L.orc_relax_finite(len(eos), _eos_arr(eos), eps, rate, dt, a.ctypes.data_as(DP), r.ctypes.data_as(DP),
+                           p.ctypes.data_as(DP))
+    return a, r, p, n
+
+
+def _energy(e, r, p):
+    if e["type"] == "SG":
+        return (p + e["gamma"] * e["pinf"]) / (e["gamma"] - 1.) / r
+    return p / (e["gamma"] - 1.) / r
+
+
+def _pressure(e, r, en):
+    if e["type"] == "SG":
+        return (e["gamma"] - 1.) * r * en - e["gamma"] * e["pinf"]
+    return (e["gamma"] - 1.) * r * en
+
+
+def _sound(e, r, p):
+    return np.sqrt(e["gamma"] * (p + e.get("pinf", 0.0)) / r)
+
+
+def _ode(eos, a0, r0, p0, mu, dt):
+    n = len(eos)
+    m = np.array(a0) * np.array(r0)
+    y0 = np.concatenate([a0, [_energy(eos[k], r0[k], p0[k]) for k in range(n)]])
+
+    def f(t, y):
+        a, en = y[:n], y[n:]
+        r = m / a
+        p = np.array([_pressure(eos[k], r[k], en[k]) for k in range(n)])
+        z = np.array([r[k] * _sound(eos[k], r[k], p[k]) for k in range(n)])
+        pI = sum(p[k] * (z.sum() - z[k]) for k in range(n)) / ((n - 1) * z.sum())
+        dp = np.array([a[k] * sum(a[j] * (p[k] - p[j]) for j in range(n) if j != k) for k in range(n)])
+        return np.concatenate([mu * dp, -mu * pI * dp / m])
+    sol = solve_ivp(f, (0.0, dt), y0, rtol=1e-12, atol=1e-14, method="LSODA")
+    y = sol.y[:, -1]
+    a = y[:n]
+    return a, m / a, np.array([_pressure(eos[k], m[k] / a[k], y[n + k]) for k in range(n)])
+
+
+CASES = [([SG_WATER, IG_AIR], [0.6, 0.4], [1000., 1.2], [2e6, 1e5]),
+         ([SG_WATER, IG_AIR], [0.05, 0.95], [1010., 5.0], [1e5, 8e5]),
+         ([SG_WATER, SG_EPOXY, IG_AIR], [0.3, 0.3, 0.4], [1000., 1185., 2.0], [5e6, 1e6, 2e5])]
+
+
+@pytest.mark.parametrize("eos,a,r,p", CASES)
+def test_one_step_matches_the_ode(eos, a, r, p):
+    # a change of alpha of 1e-9: one Euler step, accurate to second order in mu dt (the stiff
+    # water phase makes the second-order term 1 % of the change at 1e-5)
+    dt = 1e-3
+    n = len(a)
+    dpk = [a[k] * sum(a[j] * (p[k] - p[j]) for j in range(n) if j != k) for k in range(n)]
+    mu = 1e-9 / (dt * max(abs(x) for x in dpk))
+    af, rf, pf, n = _finite(eos, a, r, p, mu, dt)
+    ao, ro, po = _ode(eos, a, r, p, mu, dt)
+    da_ref = ao - np.array(a)
+    assert np.max(np.abs(da_ref)) > 1e-10
+    assert np.allclose(af - np.array(a), da_ref, rtol=1e-3, atol=1e-15), (af - a, da_ref)
+    dp_ref = po - np.array(p)
+    assert np.allclose(pf - np.array(p), dp_ref, rtol=1e-2, atol=1e-6 * np.max(np.abs(dp_ref))), (pf - p, dp_ref)
+
+
+@pytest.mark.parametrize("eos,a,r,p", CASES)
+def test_conservation_and_monotone_relaxation(eos, a, r, p):
+    m0 = np.array(a) * np.array(r)
+    e0 = sum(m0[k] * _energy(eos[k], r[k], p[k]) for k in range(len(eos)))
+    spread = []
+    for dt in (1e-7, 1e-6, 1e-5, 1e-4):
+        af, rf, pf, n = _finite(eos, a, r, p, 1.0 / max(p) * 1e2, dt)
+        assert np.allclose(af * rf, m0, rtol=1e-13, atol=0)
+        assert abs(af.sum() - 1.0) < 1e-10      # the closing Newton solve stops at abs(f) <= 1e-10
+        e1 = sum(af[k] * rf[k] * _energy(eos[k], rf[k], pf[k]) for k in range(len(eos)))
+        assert abs(e1 - e0) <= 1e-9 * abs(e0), (e1, e0)
+        spread.append(np.ptp(pf))
+    assert all(b <= a_ for a_, b in zip(spread, spread[1:])) and spread[-1] < spread[0]
+
+
+@pytest.mark.parametrize("eos,a,r,p", CASES)
+def test_fast_relaxation_ends_in_the_infinite_relaxation(eos, a, r, p):
+    L = lib()
+    af, rf, pf, n = _finite(eos, a, r, p, 1e3, 1e-3)
+    ai, ri, pi = (np.array(x, dtype=float) for x in (a, r, p))
+    it = L.orc_relax_cell(len(eos), _eos_arr(eos), 0.0, 0, ai.ctypes.data_as(DP), ri.ctypes.data_as(DP),
+                          pi.ctypes.data_as(DP), None)
+    assert 0 < it < 100
+    assert np.array_equal(af, ai) and np.array_equal(rf, ri) and np.array_equal(pf, pi)
+
+
+def test_gates():
+    # alphaNull active and a phase filling the cell: nothing happens
+    a, r, p, n = _finite([SG_WATER, IG_AIR], [1 - 1e-9, 1e-9], [1000., 1.2], [2e6, 1e5], 1.0, 1e-3, eps=1e-15)
+    assert n == 0 and a[0] == 1 - 1e-9 and p[0] == 2e6
+    # zero dt: nothing happens
+    a, r, p, n = _finite([SG_WATER, IG_AIR], [0.5, 0.5], [1000., 1.2], [2e6, 1e5], 1.0, 0.0)
+    assert n == 0 and p[1] == 1e5
