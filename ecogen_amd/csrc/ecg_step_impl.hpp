@@ -30,8 +30,9 @@ namespace ecg {
 // their form on the 512^3 case.
 // Limiter template argument LK of the step kernel: a limiter kind >= 0 fixes every
 // limiter of the case at compile time; LK_GENERIC reads the four kinds per face;
-// LK_FULL does the same and also builds second-order walls and the THINC
-// reconstruction (their code costs the other instantiations registers).
+// LK_FULL does the same and also builds second-order walls, the THINC
+// reconstruction and the finite-rate relaxation (their code costs the other
+// instantiations registers).
 constexpr int LK_GENERIC = -1;
 constexpr int LK_FULL = -2;
 
@@ -562,7 +563,7 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
     Cons<N> A;
     time_evolution<MODEL, N, D, R>(P, acc, A, own64, scale);
     build_prim<MODEL, N>(P, A, nw);
-    if (MODEL != M_EULER && P.relax && !(MODEL == M_UEQ && P.relax_mu > 0.)) {
+    if (MODEL != M_EULER && P.relax && !(MODEL == M_UEQ && LK == LK_FULL && P.relax_mu > 0.)) {
       relax_now = !relax_gated<N>(P, nw);
       if (relax_now && (LEARN || L.sink.buf != nullptr))
         spread_ok = relax_features<N>(P, nw, p_cold, spread, feat, t_lin, t_lin_d, sigma);
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
     // Relaxation: after the predictor (CellO2.cpp:128-134) and after the full step
     // (Run::advancingProcedure, Run.cpp:593).
     double fres = 0.;
-    if (MODEL == M_UEQ && P.relax_mu > 0.) {
+    if (MODEL == M_UEQ && LK == LK_FULL && P.relax_mu > 0.) {
       it = relax_finite<N>(P, nw, relax_dt);
     } else if (LEARN && spread_ok) {
       bool fell = false;
@@ -724,7 +725,7 @@ void launch_step_v(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counter
 // choice) runs the instantiation with the limiter fixed at compile time.
 template <int MODEL, int N, int D, typename R, bool LEARN>
 void launch_step_o2(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counters* cnt, hipStream_t s) {
-  bool full = P.thinc != 0;
+  bool full = P.thinc != 0 || P.relax_mu > 0.;
   for (int a = 0; a < D; a++) full = full || P.bc[2 * a] == BC_WALL || P.bc[2 * a + 1] == BC_WALL;
   const bool all_minmod = P.lim[0] == LIM_MINMOD && P.lim[1] == LIM_MINMOD && P.lim[2] == LIM_MINMOD &&
                           P.lim[3] == LIM_MINMOD;
@@ -745,6 +746,9 @@ void launch_step_t(const Params& P, const StepLaunch& Lv, Clock* clk, Counters* 
   if (Lv.stage == 0) {
     if constexpr (CAN_LEARN) {
       if (learn) { launch_step_v<MODEL, N, D, false, R, true, LK_GENERIC>(P, L, clk, cnt, s); return; }
+    }
+    if constexpr (MODEL == M_UEQ) {
+      if (P.relax_mu > 0.) { launch_step_v<MODEL, N, D, false, R, false, LK_FULL>(P, L, clk, cnt, s); return; }
     }
     launch_step_v<MODEL, N, D, false, R, false, LK_GENERIC>(P, L, clk, cnt, s);
   } else {
