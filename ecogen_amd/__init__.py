@@ -21,7 +21,7 @@ MAX_PHASES = 4
 MODELS = {"euler": 0, "pueq": 1, "ueq": 2}
 EOS_TYPES = {"IG": 0, "SG": 1}
 LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
-BCS = {"nonreflecting": 0, "wall": 1, "symmetry": 1}
+BCS = {"nonreflecting": 0, "wall": 1, "symmetry": 1, "inlettank": 2}
 DOMAINS = {"entire": 0, "halfspace": 1, "disc": 2, "sphere": 3, "rectangle": 4, "cuboid": 5}
 PRECISIONS = {"fp64": 0, "fp32": 1, "fp32_learned": 2}
 
@@ -31,12 +31,16 @@ class EcgEos(C.Structure):
                 ("eref", C.c_double)]
 
 
+class EcgTank(C.Structure):
+    _fields_ = [("p0", C.c_double), ("T0", C.c_double), ("alpha", C.c_double * MAX_PHASES)]
+
+
 class EcgConfig(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
                 ("model", C.c_int), ("nph", C.c_int), ("eos", EcgEos * MAX_PHASES), ("order", C.c_int),
                 ("lim", C.c_int * 4), ("alpha_null", C.c_int), ("relax_p", C.c_int), ("bc", C.c_int * 6),
-                ("cfl", C.c_double), ("precision", C.c_int), ("relax_rate", C.c_double)]
+                ("cfl", C.c_double), ("precision", C.c_int), ("relax_rate", C.c_double), ("tank", EcgTank * 6)]
 
 
 class EcgDomain(C.Structure):
@@ -164,6 +168,12 @@ def make_config(case):
     c.cfl = case.get("cfl", 0.8)
     c.precision = PRECISIONS[case.get("precision", "fp64")]
     c.relax_rate = float(case.get("relax_rate", 0.0))
+    # "tanks": {side: dict(p0=, T0=, alpha=[...])} for the "inletTank" sides (0..5 = X-, X+, Y-, Y+, Z-, Z+)
+    for side, t in case.get("tanks", {}).items():
+        o = c.tank[int(side)]
+        o.p0, o.T0 = float(t["p0"]), float(t["T0"])
+        for k, a in enumerate(t.get("alpha", [1.0])):
+            o.alpha[k] = float(a)
     return c
 
 
@@ -218,12 +228,14 @@ def read_case(case_dir, eos_dir=None, precision="fp64"):
                             radius=d.radius, center=tuple(d.center), lo=tuple(d.lo), len=tuple(d.len),
                             vel=tuple(d.vel), alpha=list(d.alpha[:cfg.nph]), rho=list(d.rho[:cfg.nph]),
                             p=list(d.p[:cfg.nph])))
-    bcs = ("nonreflecting", "wall")
+    bcs = ("nonreflecting", "wall", "inletTank")
+    tanks = {i: dict(p0=cfg.tank[i].p0, T0=cfg.tank[i].T0, alpha=list(cfg.tank[i].alpha[:cfg.nph]))
+             for i in range(6) if cfg.bc[i] == BCS["inlettank"]}
     return dict(name=rc.run_name.decode(), nx=cfg.nx, ny=cfg.ny, nz=cfg.nz, lx=cfg.lx, ly=cfg.ly, lz=cfg.lz,
                 model=inv(MODELS, cfg.model), eos=eos, order=cfg.order,
                 limiters=tuple(inv(LIMITERS, cfg.lim[i]) for i in range(4)), alpha_null=bool(cfg.alpha_null),
                 relax_p=bool(cfg.relax_p), relax_rate=cfg.relax_rate, bc=tuple(bcs[cfg.bc[i]] for i in range(6)),
-                cfl=cfg.cfl,
+                tanks=tanks, cfl=cfg.cfl,
                 precision=precision, domains=domains,
                 control=dict(iterations_mode=bool(rc.iterations_mode), iterations=rc.iterations,
                              iter_freq=rc.iter_freq, total_time=rc.total_time, time_freq=rc.time_freq))

@@ -260,6 +260,37 @@ static EosDev make_eos(const ecg_eos& e) {
   return q;
 }
 
+// Tank constants (BoundCondInletTank.cpp:56, 121 for the densities; ModUEq.cpp:562-569
+// and ModEuler.cpp:474-476 for the mixture density, mass fractions and total
+// enthalpy, whose zero tank velocity adds exactly nothing).
+static TankDev make_tank(const ecg_config& g, double eps, const ecg_tank& t) {
+  TankDev T;
+  std::memset(&T, 0, sizeof(T));
+  T.p0 = t.p0;
+  auto energy = [&](const ecg_eos& q, double rho, double p) {
+    if (q.type == ECG_EOS_SG) return (p + q.gamma * q.pinf) / (q.gamma - 1.) / std::max(rho, eps) + q.eref;
+    return p / (q.gamma - 1.) / std::max(rho, eps) + q.eref;
+  };
+  for (int k = 0; k < g.nph; k++) {
+    const ecg_eos& q = g.eos[k];
+    const double num = (q.type == ECG_EOS_SG) ? t.p0 + q.pinf : t.p0;
+    T.rhok0[k] = num / std::max(((q.gamma - 1.) * q.cv * t.T0), eps);
+  }
+  if (g.model == ECG_EULER) {
+    T.Yk0[0] = 1.;
+    T.H0 = energy(g.eos[0], T.rhok0[0], T.p0) + T.p0 / std::max(T.rhok0[0], eps);
+    return T;
+  }
+  double rho0 = 0.;
+  for (int k = 0; k < g.nph; k++) rho0 += t.alpha[k] * T.rhok0[k];
+  T.H0 = 0.;
+  for (int k = 0; k < g.nph; k++) {
+    T.Yk0[k] = t.alpha[k] * T.rhok0[k] / rho0;
+    T.H0 += T.Yk0[k] * (energy(g.eos[k], T.rhok0[k], T.p0) + T.p0 / std::max(T.rhok0[k], eps));
+  }
+  return T;
+}
+
 static int check_config(const ecg_config* c) {
   if (!c) return fail(ECG_E_ARG, "null config");
   if (c->nx < 2) return fail(ECG_E_ARG, "Number of cells in the x-direction has to be superior to 1");
@@ -286,9 +317,29 @@ static int check_config(const ecg_config* c) {
         return fail(ECG_E_UNSUPPORTED, "second order needs minmod/vanleer/vanalbada/superbee/mc/THINC limiters");
     }
   }
+  bool tank = false;
   for (int i = 0; i < 6; i++) {
-    if (c->bc[i] != ECG_BC_NONREFLECTING && c->bc[i] != ECG_BC_WALL) return fail(ECG_E_ARG, "unknown boundary");
+    if (c->bc[i] != ECG_BC_NONREFLECTING && c->bc[i] != ECG_BC_WALL && c->bc[i] != ECG_BC_INLET_TANK)
+      return fail(ECG_E_ARG, "unknown boundary");
+    if (c->bc[i] != ECG_BC_INLET_TANK) continue;
+    tank = true;
+    const ecg_tank& t = c->tank[i];
+    if (!(t.p0 > 0.) || !std::isfinite(t.p0) || !(t.T0 > 0.) || !std::isfinite(t.T0))
+      return fail(ECG_E_ARG, "inlet tank " + std::to_string(i) + ": p0 and T0 must be positive");
+    for (int k = 0; k < c->nph; k++)
+      if (!(c->eos[k].cv > 0.)) return fail(ECG_E_ARG, "inlet tank: the EOS of phase " + std::to_string(k) +
+                                                           " needs cv > 0 for its density at (p0, T0)");
+    if (c->model != ECG_EULER) {
+      double sum = 0.;
+      for (int k = 0; k < c->nph; k++) {
+        if (!(t.alpha[k] >= 0. && t.alpha[k] <= 1.)) return fail(ECG_E_ARG, "alpha should be in [0,1]");
+        sum += t.alpha[k];
+      }
+      if (std::fabs(sum - 1.) > 1.e-6) return fail(ECG_E_ARG, "sum of alpha should be 1");
+    }
   }
+  if (tank && c->order == 1 && c->precision == ECG_FP32_LEARNED)
+    return fail(ECG_E_UNSUPPORTED, "the learned warm start runs first-order cases without inlet tanks");
   if (!(c->cfl > 0)) return fail(ECG_E_ARG, "CFL must be positive");
   if (c->precision != ECG_FP64 && c->precision != ECG_FP32_RELAX64 && c->precision != ECG_FP32_LEARNED)
     return fail(ECG_E_ARG, "unknown precision form");
@@ -357,6 +408,8 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   P.thinc_cosh = std::cosh(1.6);
   P.thinc_tanh = std::tanh(1.6);
   for (int k = 0; k < g.nph; k++) P.eos[k] = make_eos(g.eos[k]);
+  for (int i = 0; i < 6; i++)
+    if (g.bc[i] == ECG_BC_INLET_TANK) P.tank[i] = make_tank(g, P.eps, g.tank[i]);
   int tex[3];
   tile_extent(c->D, c->esize, tex);
   for (int a = 0; a < 3; a++) P.tiles[a] = (n3[a] + tex[a] - 1) / tex[a];
@@ -586,6 +639,9 @@ static int settle(ecg_ctx* c) {
   return ECG_OK;
 }
 
+// Errors::errorMessage of ModEuler.cpp:487 / ModUEq.cpp:586 (the reference stops there).
+static const char* const TANK_FAIL = "solveRiemannInletTank not converged (50 Newton iterations)";
+
 // Host sum of the counter slots (fixed order).
 static Counters sum_counters(const ecg_ctx* c) {
   Counters s;
@@ -598,6 +654,7 @@ static Counters sum_counters(const ecg_ctx* c) {
     s.warm_calls += h.warm_calls;
     s.warm_fallbacks += h.warm_fallbacks;
     s.resid_sum += h.resid_sum;
+    s.tank_fail += h.tank_fail;
   }
   return s;
 }
@@ -623,7 +680,8 @@ int ecg_step(ecg_ctx* c, double dt, double* dtmax_out, ecg_step_stats* st) {
   int rc = read_clock(c);
   if (rc != ECG_OK) return rc;
   Clock k = *c->hclk;
-  const int64_t fails_before = (int64_t)sum_counters(c).newton_fail;
+  const Counters before = sum_counters(c);
+  const int64_t fails_before = (int64_t)before.newton_fail;
   k.dt = dt;
   k.done = 0;
   k.max_iter = 0;
@@ -637,7 +695,9 @@ int ecg_step(ecg_ctx* c, double dt, double* dtmax_out, ecg_step_stats* st) {
   c->pending = 0;
   if (dtmax_out) *dtmax_out = c->hclk->dt_max;
   fill_stats(c, st);
-  const int64_t nfail = (int64_t)sum_counters(c).newton_fail - fails_before;
+  const Counters after = sum_counters(c);
+  if (after.tank_fail != before.tank_fail) return fail(ECG_E_STATE, TANK_FAIL);
+  const int64_t nfail = (int64_t)after.newton_fail - fails_before;
   return nfail > 0 ? (int)nfail : ECG_OK;
 }
 
@@ -646,6 +706,7 @@ int ecg_step(ecg_ctx* c, double dt, double* dtmax_out, ecg_step_stats* st) {
 static int run_clock(ecg_ctx* c, const Clock& k, int64_t max_steps, ecg_step_stats* st) {
   const int start = c->cur;
   const int64_t it0 = k.iteration;
+  const unsigned long long tank_fail0 = sum_counters(c).tank_fail;
   int rc = set_clock(c, k);
   if (rc != ECG_OK) return rc;
   int64_t enq = 0;
@@ -666,6 +727,7 @@ static int run_clock(ecg_ctx* c, const Clock& k, int64_t max_steps, ecg_step_sta
   c->cur = (int)((start + done_steps) % 2);
   c->pending = 0;
   fill_stats(c, st);
+  if (sum_counters(c).tank_fail != tank_fail0) return fail(ECG_E_STATE, TANK_FAIL);
   return ECG_OK;
 }
 

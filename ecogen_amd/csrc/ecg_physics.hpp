@@ -27,7 +27,7 @@ namespace ecg {
 
 enum { M_EULER = 0, M_PUEQ = 1, M_UEQ = 2 };
 enum { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_VANALBADA = 3, LIM_SUPERBEE = 4, LIM_MC = 5, LIM_THINC = 6 };
-enum { BC_NONREFLECTING = 0, BC_WALL = 1 };
+enum { BC_NONREFLECTING = 0, BC_WALL = 1, BC_TANK = 2 };
 
 struct EosDev {
   int sg;          // 1: stiffened gas, 0: ideal gas
@@ -39,6 +39,15 @@ struct EosDev {
   double mixg;     // gamma*pinf/(gamma-1) (IG: 0)   EosSG.cpp:208-214
   double mixinv;   // 1/(gamma-1)
   double pclamp;   // SG pressure floor -(1-1e-15)*pinf + 1e-15
+};
+
+// Stagnation tank of an inletTank side (BoundCondInletTank.cpp:37-165) with the
+// constants the model's tank solver derives from it (ModUEq.cpp:562-569,
+// ModEuler.cpp:474-476), computed on the host: phase densities at (p0, T0), the
+// tank's mass fractions and its total enthalpy.
+struct TankDev {
+  double p0, H0;
+  double rhok0[4], Yk0[4];
 };
 
 struct Params {
@@ -63,6 +72,7 @@ struct Params {
   double relax_mu;          // > 0: finite-rate pressure relaxation with this rate (VELOCITYEQ)
   int thinc;                // a volume-fraction limiter is THINC (two phases)
   double thinc_cosh, thinc_tanh;   // cosh(1.6), tanh(1.6) of the host's libm
+  TankDev tank[6];          // sides whose bc is BC_TANK
 };
 
 // Device clock: lagged time step kept on the GPU (Run.cpp:385-498).
@@ -85,7 +95,8 @@ struct Counters {
   unsigned long long warm_fallbacks;  // of those, cold restarts after WARM_MAX_IT iterations
   double resid_sum;                   // sum over relaxed cells of |f| at exit
   uint64_t dtmax_bits;                // min of the step's dtMax (non-negative doubles order as integers)
-  uint64_t pad[9];
+  unsigned long long tank_fail;       // inlet-tank Newton solves that spent 50 iterations
+  uint64_t pad[8];
 };
 static_assert(sizeof(Counters) == 128, "one counter slot per 128-byte line");
 
@@ -149,6 +160,31 @@ __device__ __forceinline__ double eos_rho_pfinal(const EosDev& q, double eps, do
   double rhof = rho0 * num / (q.sg ? nonzero(denom, eps) : dmax(denom, eps));
   drhodp = rho0 * q.g * (q.sg ? (p0 + q.pinf) : p0) / dmax(denom * denom, eps);
   return rhof;
+}
+
+// computeDensityHugoniot (EosSG.cpp:152-159, EosIG.cpp:150-157); FP64 only (tank boundary).
+__device__ __forceinline__ double eos_rho_hugoniot(const EosDev& q, double eps, double p0, double rho0, double pf) {
+  const double P0 = q.sg ? p0 + q.pinf : p0, Pf = q.sg ? pf + q.pinf : pf;
+  const double num = (q.g + 1.) * Pf + (q.g - 1.) * P0;
+  const double denom = (q.g - 1.) * Pf + (q.g + 1.) * P0;
+  return rho0 * num / dmax(denom, eps);
+}
+// computeDensityIsentropic (EosSG.cpp:143-148, EosIG.cpp:141-146) with d rho / dp.
+__device__ __forceinline__ double eos_rho_isentropic(const EosDev& q, double eps, double p0, double rho0, double pf,
+                                                     double& drhodp) {
+  const double P0 = q.sg ? p0 + q.pinf : p0, Pf = q.sg ? pf + q.pinf : pf;
+  const double rhof = rho0 * pow(Pf / dmax(P0, eps), 1. / q.g);
+  drhodp = rhof / dmax(q.g * Pf, eps);
+  return rhof;
+}
+// computeEnthalpyIsentropic (EosSG.cpp:174-181, EosIG.cpp:172-179) with d h / dp.
+__device__ __forceinline__ double eos_h_isentropic(const EosDev& q, double eps, double p0, double rho0, double pf,
+                                                   double& rhof, double& dhdp) {
+  double drho;
+  rhof = eos_rho_isentropic(q, eps, p0, rho0, pf, drho);
+  const double Pf = q.sg ? pf + q.pinf : pf;
+  dhdp = q.g / (q.g - 1.) * (rhof - Pf * drho) / dmax(rhof * rhof, eps);
+  return q.g * Pf / (q.g - 1.) / dmax(rhof, eps) + q.eref;
 }
 
 // ---------------------------------------------------------------------------
@@ -505,6 +541,188 @@ __device__ __forceinline__ void boundary_flux(const Params& P, const Cell<N, R>&
   } else {
     face_flux<MODEL, N, FR, R>(P, B, EB, B, EB, dtm, F);
   }
+}
+
+// Inlet-tank boundary (ModEuler::solveRiemannInletTank, ModEuler.cpp:413-545;
+// ModUEq::solveRiemannInletTank, ModUEq.cpp:496-673, shared by PRESSUREVELOCITYEQ)
+// on the derived buffer cell B with face-frame velocity lu (normal out of the
+// domain); the flux is returned in the face frame.  FP64 in both arithmetic
+// forms: the pressure Newton works to 1e-3 (Euler) / 1e-2 (mixture) in velocity
+// and steps p below p0 by 1e-6, which single precision cannot resolve.  Returns
+// the Newton iterations of the tank-inflow branch, or -1 once 50 are spent (the
+// reference stops with an error there).
+template <int MODEL, int N>
+__device__ __noinline__ int tank_flux(const Params& P, const TankDev& T, const Cell<N, double>& B, const double* lu,
+                                      double& dtm, Flux<N, double>& F) {
+  const double eps = P.eps, p0 = T.p0, lcfl = P.lcfl;
+  int iteration = 0;
+  if constexpr (MODEL == M_EULER) {
+    const EosDev& q = P.eos[0];
+    const double uL = lu[0], vL = lu[1], wL = lu[2], pL = B.p[0], rhoL = B.r[0], zL = rhoL * B.c[0];
+    double uStar, rhoStar, pStar = p0, vStar, wStar, drho, mL, vmv0;
+    double v = 1. / eos_rho_isentropic(q, eps, pL, rhoL, pStar, drho);
+    vmv0 = v - 1. / rhoL;
+    mL = (fabs(vmv0) > 1e-10) ? sqrt((pL - p0) / vmv0) : zL;
+    double sL = uL - mL / rhoL;
+    cfl_bound<double>(dtm, sL, lcfl);
+    const double u = uL + mL * vmv0;
+    if (sL >= 0.) {
+      uStar = uL; pStar = pL; rhoStar = rhoL; vStar = vL; wStar = wL;
+    } else if (u >= -1e-3) {
+      uStar = u; pStar = p0; rhoStar = 1. / v; vStar = vL; wStar = wL;
+    } else {
+      double p = 0.5 * p0, f = 0., df = 1., uStarR = 0., uStarL = 0.;
+      do {
+        p -= f / df;
+        iteration++;
+        if (iteration > 50) return -1;
+        p = eos_clamp_p<double>(q, p);
+        if (p > p0) p = p0 - 1e-6;
+        double dh;
+        const double h = eos_h_isentropic(q, eps, p0, T.rhok0[0], p, rhoStar, dh);
+        uStarR = -sqrt(2. * (T.H0 - h));
+        const double duStarR = -dh / uStarR;
+        v = 1.0 / eos_rho_isentropic(q, eps, pL, rhoL, p, drho);
+        const double dv = -v * v * drho;
+        vmv0 = v - 1. / rhoL;
+        double dmL = 0.;
+        if (fabs(vmv0) > 1e-10) {
+          mL = sqrt((pL - p) / vmv0);
+          dmL = 0.5 * (-vmv0 + (p - pL) * dv) / (vmv0 * vmv0) / mL;
+        } else {
+          mL = zL;
+        }
+        sL = uL - mL / rhoL;
+        cfl_bound<double>(dtm, sL, lcfl);
+        uStarL = uL + mL * vmv0;
+        const double duStarL = dmL * vmv0 + mL * dv;
+        f = uStarR - uStarL;
+        df = duStarR - duStarL;
+      } while (fabs(f) > 1e-3);
+      pStar = p;
+      uStar = 0.5 * (uStarL + uStarR);
+      vStar = 0.;
+      wStar = 0.;
+    }
+    const double eStar = eos_energy<double>(q, eps, rhoStar, pStar);
+    F.f.mass[0] = rhoStar * uStar;
+    F.f.mom[0] = rhoStar * uStar * uStar + pStar;
+    F.f.mom[1] = rhoStar * uStar * vStar;
+    F.f.mom[2] = rhoStar * uStar * wStar;
+    F.f.energM = (rhoStar * (eStar + 0.5 * (uStar * uStar + vStar * vStar + wStar * wStar)) + pStar) * uStar;
+    F.sM = uStar;
+    F.uStar = uStar;
+  } else {
+    double rks[N], Yks[N];
+    const double uL = lu[0], pL = B.pm, rhoL = B.rho;
+    // Wood sound speed of the mixture (MixUEq.cpp:152-159)
+    double cL = 0.;
+#pragma unroll
+    for (int k = 0; k < N; k++) cL += B.a[k] / dmax(B.r[k] * B.c[k] * B.c[k], eps);
+    cL = 1. / sqrt(rhoL * cL);
+    const double zL = rhoL * cL;
+    double pStar = p0, vStar = 0., mL, vmv0, uStar, rhoStar, uyStar, uzStar;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      rks[k] = eos_rho_hugoniot(P.eos[k], eps, B.p[k], B.r[k], pStar);
+      vStar += B.a[k] * B.r[k] / rhoL / dmax(rks[k], eps);
+    }
+    vmv0 = vStar - 1. / rhoL;
+    mL = (fabs(vmv0) > 1e-10) ? sqrt((pL - pStar) / vmv0) : zL;
+    double sL = uL - mL / rhoL;
+    cfl_bound<double>(dtm, sL, lcfl);
+    double sM = uL + mL * vmv0;
+    if (sL >= 0.) {
+      uStar = uL;
+      pStar = pL;
+#pragma unroll
+      for (int k = 0; k < N; k++) { rks[k] = B.r[k]; Yks[k] = B.a[k] * B.r[k] / rhoL; }
+      rhoStar = rhoL;
+      uyStar = lu[1];
+      uzStar = lu[2];
+    } else if (sM >= -1e-3) {
+      uStar = sM;
+      pStar = p0;
+#pragma unroll
+      for (int k = 0; k < N; k++) Yks[k] = B.a[k] * B.r[k] / rhoL;
+      rhoStar = 1. / vStar;
+      uyStar = lu[1];
+      uzStar = lu[2];
+    } else {
+      double p = 0.5 * p0, f = 0., df = 1., uStarR = 0., uStarL = 0.;
+      do {
+        p -= f / df;
+        iteration++;
+#pragma unroll
+        for (int k = 0; k < N; k++) p = eos_clamp_p<double>(P.eos[k], p);
+        if (p > p0) p = p0 - 1e-6;
+        if (iteration > 50) return -1;
+        // Tank side: isentropes from (p0, rhok0) and H = H0.
+        uStarR = T.H0;
+        double duStarR = 0.;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+          double dhk;
+          const double hk = eos_h_isentropic(P.eos[k], eps, p0, T.rhok0[k], p, rks[k], dhk);
+          uStarR -= T.Yk0[k] * hk;
+          duStarR -= T.Yk0[k] * dhk;
+        }
+        uStarR = -sqrt(2. * uStarR);
+        duStarR = duStarR / uStarR;
+        // Cell side: isentropes of every phase.
+        double vStarL = 0., dvStarL = 0.;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+          double drhok;
+          const double rhok = eos_rho_isentropic(P.eos[k], eps, B.p[k], B.r[k], p, drhok);
+          const double YkL = B.a[k] * B.r[k] / rhoL;
+          vStarL += YkL / dmax(rhok, eps);
+          dvStarL -= YkL / dmax(rhok * rhok, eps) * drhok;
+        }
+        vmv0 = vStarL - 1. / rhoL;
+        double dmL = 0.;
+        if (fabs(vmv0) > 1e-10) {
+          mL = sqrt((pL - p) / vmv0);
+          dmL = 0.5 * (-vmv0 + (p - pL) * dvStarL) / (vmv0 * vmv0) / mL;
+        } else {
+          mL = zL;
+        }
+        sL = uL - mL / rhoL;
+        cfl_bound<double>(dtm, sL, lcfl);
+        uStarL = uL + mL * vmv0;
+        const double duStarL = dmL * vmv0 + mL * dvStarL;
+        f = uStarR - uStarL;
+        df = duStarR - duStarL;
+      } while (fabs(f) > 1e-2);
+      pStar = p;
+      uStar = 0.5 * (uStarL + uStarR);
+      rhoStar = 0.;
+#pragma unroll
+      for (int k = 0; k < N; k++) { Yks[k] = T.Yk0[k]; rhoStar += Yks[k] / dmax(rks[k], eps); }
+      rhoStar = 1. / rhoStar;
+      uyStar = 0.;
+      uzStar = 0.;
+      sM = uStar;
+    }
+    // Flux (ModUEq.cpp:643-665)
+    double EStar = 0.5 * (uStar * uStar + uyStar * uyStar + uzStar * uzStar);
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const double ek = eos_energy<double>(P.eos[k], eps, rks[k], pStar);
+      EStar += Yks[k] * ek;
+      const double akStar = Yks[k] * rhoStar / dmax(rks[k], eps);
+      F.f.alpha[k] = akStar * sM;
+      F.f.mass[k] = akStar * uStar * rks[k];
+      F.f.energ[k] = F.f.mass[k] * ek;
+    }
+    F.f.mom[0] = rhoStar * uStar * uStar + pStar;
+    F.f.mom[1] = rhoStar * uStar * uyStar;
+    F.f.mom[2] = rhoStar * uStar * uzStar;
+    F.f.energM = (rhoStar * EStar + pStar) * uStar;
+    F.sM = sM;
+    F.uStar = uStar;
+  }
+  return iteration;
 }
 
 // ---------------------------------------------------------------------------

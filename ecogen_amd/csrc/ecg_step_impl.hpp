@@ -403,13 +403,53 @@ __device__ __forceinline__ void boundary_face(const Params& P, const Cell<N, R>&
   sub_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
 }
 
+// Inlet-tank face of side `side` (BoundCondInletTank::solveRiemannBoundary,
+// BoundCondInletTank.cpp:209-212): the buffer cell is the stage state of the own
+// cell, as for every first-order boundary class; the tank solver runs in FP64.
+// Sets bad when the tank Newton did not converge (the face then adds nothing).
+template <int MODEL, int N, int D, int FR, typename R>
+__device__ __forceinline__ void tank_face(const Params& P, const Cell<N, R>& src, int side, R coef, const R* an,
+                                          const R* pn, R& dtm, ConsAcc<N, R>& acc, int& bad) {
+  Cell<N, double> b;
+#pragma unroll
+  for (int k = 0; k < N; k++) { b.a[k] = src.a[k]; b.r[k] = src.r[k]; b.p[k] = src.p[k]; }
+  b.u[0] = src.u[0]; b.u[1] = src.u[1]; b.u[2] = src.u[2];
+  derive<MODEL, N, double>(P, b, MODEL == M_EULER);
+  double lu[3];
+  to_local<FR, double>(b.u, lu);
+  double d64 = (double)dtm;
+  Flux<N, double> F64 = {};
+  if (tank_flux<MODEL, N>(P, P.tank[side], b, lu, d64, F64) < 0) { bad = 1; return; }
+  const double loc[3] = {F64.f.mom[0], F64.f.mom[1], F64.f.mom[2]};
+  to_global<FR, double>(loc, F64.f.mom);
+  dtm = dmin<R>(dtm, R(d64));
+  Flux<N, R> F;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    F.f.alpha[k] = R(F64.f.alpha[k]); F.f.mass[k] = R(F64.f.mass[k]); F.f.energ[k] = R(F64.f.energ[k]);
+  }
+  F.f.mom[0] = R(F64.f.mom[0]); F.f.mom[1] = R(F64.f.mom[1]); F.f.mom[2] = R(F64.f.mom[2]);
+  F.f.energM = R(F64.f.energM);
+  F.sM = R(F64.sM);
+  F.uStar = R(F64.uStar);
+  sub_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
+}
+
 // Boundary face of axis AX, side SIDE (0: minus, 1: plus) of the own cell c.  A
-// second-order wall first extrapolates the cell's stage state to the face.
+// second-order wall first extrapolates the cell's stage state to the face; an
+// inlet tank solves its own Riemann problem (both only in the LK_FULL kernels).
 template <int MODEL, int N, int D, bool O2, int AX, int SIDE, int FR, typename R, int LK>
 __device__ __forceinline__ void boundary_face_at(const Params& P, const StepLaunchT<R>& L, const Cell<N, R>& src,
-                                                 int64_t c, const R* an, const R* pn, R& dtm, ConsAcc<N, R>& acc) {
+                                                 int64_t c, const R* an, const R* pn, R& dtm, ConsAcc<N, R>& acc,
+                                                 int& tank_bad) {
   const int bc = P.bc[2 * AX + SIDE];
   const R coef = R(P.coef[AX]);
+  if constexpr (LK == LK_FULL) {
+    if (bc == BC_TANK) {
+      tank_face<MODEL, N, D, FR, R>(P, src, 2 * AX + SIDE, coef, an, pn, dtm, acc, tank_bad);
+      return;
+    }
+  }
   if (!O2 || LK != LK_FULL || bc != BC_WALL) {
     boundary_face<MODEL, N, D, FR, R>(P, src, bc, coef, an, pn, dtm, acc);
     return;
@@ -523,20 +563,22 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
     const bool on_bnd = (i == 0) || (i == P.nx - 1) || (D >= 2 && (j == 0 || j == P.ny - 1)) ||
                         (D >= 3 && (k == 0 || k == P.nz - 1));
     if (on_bnd) {
+      int tank_bad = 0;
       // Boundary faces see the stage's own cell: W^n (first order / predictor) or W* (corrector).
       Cell<N, R> src;
       load_prim<MODEL, N, D>(P, (O2 && L.stage == 2) ? L.wsrc : L.wn, c, src);
       derive<MODEL, N, R>(P, src, false);
-      if (i == 0) boundary_face_at<MODEL, N, D, O2, 0, 0, FR_XM, R, LK>(P, L, src, c, an, pn, dtm, acc);
-      if (i == P.nx - 1) boundary_face_at<MODEL, N, D, O2, 0, 1, FR_XP, R, LK>(P, L, src, c, an, pn, dtm, acc);
+      if (i == 0) boundary_face_at<MODEL, N, D, O2, 0, 0, FR_XM, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
+      if (i == P.nx - 1) boundary_face_at<MODEL, N, D, O2, 0, 1, FR_XP, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
       if (D >= 2) {
-        if (j == 0) boundary_face_at<MODEL, N, D, O2, 1, 0, FR_YM, R, LK>(P, L, src, c, an, pn, dtm, acc);
-        if (j == P.ny - 1) boundary_face_at<MODEL, N, D, O2, 1, 1, FR_YP, R, LK>(P, L, src, c, an, pn, dtm, acc);
+        if (j == 0) boundary_face_at<MODEL, N, D, O2, 1, 0, FR_YM, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
+        if (j == P.ny - 1) boundary_face_at<MODEL, N, D, O2, 1, 1, FR_YP, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
       }
       if (D >= 3) {
-        if (k == 0) boundary_face_at<MODEL, N, D, O2, 2, 0, FR_ZM, R, LK>(P, L, src, c, an, pn, dtm, acc);
-        if (k == P.nz - 1) boundary_face_at<MODEL, N, D, O2, 2, 1, FR_ZP, R, LK>(P, L, src, c, an, pn, dtm, acc);
+        if (k == 0) boundary_face_at<MODEL, N, D, O2, 2, 0, FR_ZM, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
+        if (k == P.nz - 1) boundary_face_at<MODEL, N, D, O2, 2, 1, FR_ZP, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
       }
+      if (LK == LK_FULL && tank_bad) atomicAdd(&cnt[blockIdx.x % ATOMIC_SLOTS].tank_fail, 1ull);
     }
 
     // FP64 cell update from the time-n primitives.
@@ -720,12 +762,21 @@ void launch_step_v(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counter
   hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
 }
 
-// Second order: a case with a wall or a THINC limiter runs the LK_FULL
-// instantiation; otherwise a case whose four limiters are all minmod (the common
-// choice) runs the instantiation with the limiter fixed at compile time.
+// Whether a case has an inlet-tank side among its D axes.
+template <int D>
+bool has_tank(const Params& P) {
+  for (int a = 0; a < 2 * D; a++)
+    if (P.bc[a] == BC_TANK) return true;
+  return false;
+}
+
+// Second order: a case with a wall, an inlet tank, a THINC limiter or the
+// finite-rate relaxation runs the LK_FULL instantiation; otherwise a case whose
+// four limiters are all minmod (the common choice) runs the instantiation with
+// the limiter fixed at compile time.
 template <int MODEL, int N, int D, typename R, bool LEARN>
 void launch_step_o2(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counters* cnt, hipStream_t s) {
-  bool full = P.thinc != 0 || P.relax_mu > 0.;
+  bool full = P.thinc != 0 || P.relax_mu > 0. || has_tank<D>(P);
   for (int a = 0; a < D; a++) full = full || P.bc[2 * a] == BC_WALL || P.bc[2 * a + 1] == BC_WALL;
   const bool all_minmod = P.lim[0] == LIM_MINMOD && P.lim[1] == LIM_MINMOD && P.lim[2] == LIM_MINMOD &&
                           P.lim[3] == LIM_MINMOD;
@@ -747,8 +798,12 @@ void launch_step_t(const Params& P, const StepLaunch& Lv, Clock* clk, Counters* 
     if constexpr (CAN_LEARN) {
       if (learn) { launch_step_v<MODEL, N, D, false, R, true, LK_GENERIC>(P, L, clk, cnt, s); return; }
     }
-    if constexpr (MODEL == M_UEQ) {
-      if (P.relax_mu > 0.) { launch_step_v<MODEL, N, D, false, R, false, LK_FULL>(P, L, clk, cnt, s); return; }
+    // First order: inlet tanks and the finite-rate relaxation live in LK_FULL
+    // (a learned first-order case has no tank side: the configuration check
+    // refuses it).
+    if (has_tank<D>(P) || (MODEL == M_UEQ && P.relax_mu > 0.)) {
+      launch_step_v<MODEL, N, D, false, R, false, LK_FULL>(P, L, clk, cnt, s);
+      return;
     }
     launch_step_v<MODEL, N, D, false, R, false, LK_GENERIC>(P, L, clk, cnt, s);
   } else {

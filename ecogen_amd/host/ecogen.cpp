@@ -359,6 +359,65 @@ void read_state(const XmlElement& st, const CaseDescription& c, const std::strin
   }
 }
 
+// BoundCondInletTank::BoundCondInletTank (BoundCondInletTank.cpp:37-165): the
+// tank's p0 and T0 and, for several phases, one <dataFluid EOS=...> per phase
+// under <fluidsProp> giving either every alpha or every massFrac (normalised by
+// their sum; mass fractions become volume fractions through the phase densities
+// at (p0, T0)).
+ecg_tank read_tank(const XmlElement& b, const CaseDescription& c, const std::string& file) {
+  ecg_tank t{};
+  const XmlElement& d = need(b.child("dataInletTank"), "dataInletTank", file);
+  t.p0 = num(d, "p0", file);
+  t.T0 = num(d, "T0", file);
+  const int n = c.model.numberPhases;
+  if (n == 1) {
+    t.alpha[0] = 1.;
+    return t;
+  }
+  const XmlElement& fp = need(b.child("fluidsProp"), "fluidsProp", file);
+  double ak[ECG_MAX_PHASES] = {}, yk[ECG_MAX_PHASES] = {}, rk[ECG_MAX_PHASES] = {};
+  bool hasAlpha = false, hasMass = false;
+  int nf = 0;
+  for (const XmlElement* f : fp.children_named("dataFluid")) {
+    nf++;
+    const std::string* en = f->attr("EOS");
+    if (!en) bad(file, "<dataFluid> of the tank needs attribute EOS");
+    int k = 0;
+    while (k < n && c.eos[k].name != *en) k++;
+    if (k == n) bad(file, "EOS " + *en + " unknown");
+    if (has(*f, "alpha")) {
+      if (hasMass) bad(file, "only one of following is required : alpha, massFrac");
+      hasAlpha = true;
+      ak[k] = num(*f, "alpha", file);
+    }
+    if (has(*f, "massFrac")) {
+      if (hasAlpha) bad(file, "only one of following is required : alpha, massFrac");
+      hasMass = true;
+      yk[k] = num(*f, "massFrac", file);
+    }
+  }
+  if (nf != n) bad(file, "state Tank: one <dataFluid> per phase is required");
+  double* v = hasAlpha ? ak : yk;
+  const char* what = hasAlpha ? "alpha" : "massFrac";
+  if (!hasAlpha && !hasMass) bad(file, "One of following is required : alpha, massFrac");
+  double sum = 0.;
+  for (int k = 0; k < n; k++) {
+    if (v[k] < 0. || v[k] > 1.) bad(file, std::string(what) + " should be in [0,1]");
+    sum += v[k];
+  }
+  if (std::fabs(sum - 1.) > 1.e-6) bad(file, std::string("sum of ") + what + " should be 1");
+  for (int k = 0; k < n; k++) v[k] /= sum;
+  for (int k = 0; k < n; k++) rk[k] = c.eos[k].computeDensity(t.p0, t.T0);
+  if (hasMass) {
+    double rhoMel = 0.;
+    for (int k = 0; k < n; k++) rhoMel += yk[k] / rk[k];
+    rhoMel = 1.0 / rhoMel;
+    for (int k = 0; k < n; k++) ak[k] = rhoMel * yk[k] / rk[k];
+  }
+  for (int k = 0; k < n; k++) t.alpha[k] = ak[k];
+  return t;
+}
+
 void read_initial_conditions(const std::string& file, CaseDescription& c) {
   auto doc = load_xml(file);
   if (doc->name != "CI") bad(file, "root element must be <CI>");
@@ -436,7 +495,10 @@ void read_initial_conditions(const std::string& file, CaseDescription& c) {
     const std::string u = upper(*t);
     if (u == "NONREFLECTING") c.config.bc[i] = ECG_BC_NONREFLECTING;
     else if (u == "SYMMETRY" || u == "WALL") c.config.bc[i] = ECG_BC_WALL;
-    else unsupported(file, "boundary condition " + *t + " is not built");
+    else if (u == "INLETTANK") {
+      c.config.bc[i] = ECG_BC_INLET_TANK;
+      c.config.tank[i] = read_tank(*b, c, file);
+    } else unsupported(file, "boundary condition " + *t + " is not built");
     i++;
   }
   for (; i < 6; i++) c.config.bc[i] = ECG_BC_NONREFLECTING;

@@ -46,7 +46,9 @@ enum ecg_eos_type { ECG_EOS_IG = 0, ECG_EOS_SG = 1 };
  * profile's face value (LimiterTHINC.cpp, CellInterfaceO2Cartesian.cpp:133-180). */
 enum ecg_limiter { ECG_LIM_NONE = 0, ECG_LIM_MINMOD = 1, ECG_LIM_VANLEER = 2, ECG_LIM_VANALBADA = 3,
                    ECG_LIM_SUPERBEE = 4, ECG_LIM_MC = 5, ECG_LIM_THINC = 6 };
-enum ecg_bc { ECG_BC_NONREFLECTING = 0, ECG_BC_WALL = 1 };
+/* ECG_BC_INLET_TANK: subsonic inflow from a stagnation tank (BoundCondInletTank.cpp,
+ * ModEuler.cpp:413-545, ModUEq.cpp:496-673) with the state of ecg_config.tank[side]. */
+enum ecg_bc { ECG_BC_NONREFLECTING = 0, ECG_BC_WALL = 1, ECG_BC_INLET_TANK = 2 };
 /* Arithmetic of the hot path:
  *   ECG_FP64          plain double precision, the parity path
  *   ECG_FP32_RELAX64  fluxes, reconstruction and update in float with compensated
@@ -70,6 +72,14 @@ typedef struct {
   double gamma, pinf, cv, eref;
 } ecg_eos;
 
+/* Tank of an inletTank boundary (BoundCondInletTank.cpp:37-165): stagnation pressure
+ * and temperature, and the tank's volume fractions (summing to one; Euler: alpha[0] = 1).
+ * The phase densities are each EOS's computeDensity(p0, T0). */
+typedef struct {
+  double p0, T0;
+  double alpha[ECG_MAX_PHASES];
+} ecg_tank;
+
 /* What Input::readInputXML assembles for the hot path (Input.cpp:77-1193). */
 typedef struct {
   int nx, ny, nz;         /* cells per axis; nx > 1, nz > 1 requires ny > 1 (Input.cpp:459-467) */
@@ -86,6 +96,7 @@ typedef struct {
   int precision;          /* ecg_precision */
   double relax_rate;      /* > 0 (VELOCITYEQ with relax_p): <relaxation type="P" speed="finite" rate=...
                              solver="Euler"> (RelaxationPFinite) instead of the infinite relaxation */
+  ecg_tank tank[6];       /* stagnation state of each ECG_BC_INLET_TANK side (ignored elsewhere) */
 } ecg_config;
 
 /* One initial-condition domain with its state (GeometricalDomain + Phase/Mixture states). */

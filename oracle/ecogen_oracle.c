@@ -103,6 +103,38 @@ double orc_eos_rho_hugoniot(const orc_eos* q, double eps, double p0, double rho0
   }
 }
 
+double orc_eos_rho_isentropic(const orc_eos* q, double eps, double p0, double rho0, double pf, double* drhodp)
+{
+  double rhof;
+  if (q->type == ORC_EOS_SG) {
+    /* EosSG.cpp:143-148 */
+    rhof = rho0 * pow((pf + q->pinf) / omax((p0 + q->pinf), eps), 1. / q->gamma);
+    if (drhodp) *drhodp = rhof / omax((q->gamma * (pf + q->pinf)), eps);
+    return rhof;
+  }
+  /* EosIG.cpp:141-146 */
+  rhof = rho0 * pow(pf / omax(p0, eps), 1. / q->gamma);
+  if (drhodp) *drhodp = rhof / omax((q->gamma * pf), eps);
+  return rhof;
+}
+
+double orc_eos_h_isentropic(const orc_eos* q, double eps, double p0, double rho0, double pf, double* dhdp)
+{
+  /* EosSG.cpp:174-181 / EosIG.cpp:172-179 (P = pf + pinf, or pf) */
+  double drho, rhof = orc_eos_rho_isentropic(q, eps, p0, rho0, pf, &drho);
+  double P = (q->type == ORC_EOS_SG) ? (pf + q->pinf) : pf;
+  double h = q->gamma * P / (q->gamma - 1.) / omax(rhof, eps) + q->eref;
+  if (dhdp) *dhdp = q->gamma / (q->gamma - 1.) * (rhof - P * drho) / omax((rhof * rhof), eps);
+  return h;
+}
+
+double orc_eos_rho_pt(const orc_eos* q, double eps, double p, double T)
+{
+  /* EosSG.cpp:87-90 / EosIG.cpp:85-88 (computeDensity) */
+  if (q->type == ORC_EOS_SG) return (p + q->pinf) / omax(((q->gamma - 1.) * q->cv * T), eps);
+  return p / omax(((q->gamma - 1.) * q->cv * T), eps);
+}
+
 double orc_eos_clamp_p(const orc_eos* q, double p)
 {
   /* EosSG.cpp:253-256 / EosIG.cpp:251-254 (verifyAndModifyPressure) */
@@ -194,6 +226,12 @@ typedef struct {
   double u[3];
 } slope_t;
 
+/* Stagnation state of an inletTank side and the constants derived from it (tank_setup). */
+typedef struct {
+  double p0, rho0, H0;
+  double ak0[MAXPH], rhok0[MAXPH], Yk0[MAXPH];
+} tank_t;
+
 struct orc_ctx {
   orc_config cfg;
   int D, n;                 /* dimensions, phases */
@@ -210,6 +248,8 @@ struct orc_ctx {
   slope_t* slopes;          /* per interior face slopes (second order) */
   long long nface_int[3];
   long long face_off[3];
+  tank_t tk[6];             /* inletTank sides */
+  int tank_fail;            /* tank Newton solves of the current step that spent 50 iterations */
 };
 
 static const orc_eos* EOSK(const orc_ctx* c, int k) { return &c->cfg.eos[k]; }
@@ -462,6 +502,220 @@ static void wall_ueq(const cst* Lc, double dxL, double* dtMax, flux_t* F)
   F->f.mom[0] = pStar;
   F->sM = 0.;
   F->uStar = 0.;
+}
+
+/* Stagnation state of an inletTank side: BoundCondInletTank.cpp:37-165 (rhok0 = computeDensity(p0, T0)),
+   and the constants ModUEq::solveRiemannInletTank derives from it (ModUEq.cpp:562-569): rho0 from
+   MixUEq::computeDensity (MixUEq.cpp:88-96), Yk0 and H0 (computeTotalEnthalpy with zero velocity,
+   Eos.cpp:67-77, which adds exactly nothing); Euler: H0 of the single phase (ModEuler.cpp:474-476). */
+static void tank_setup(int model, int n, const orc_eos* eos, double eps, const orc_tank* t, tank_t* T)
+{
+  int k;
+  memset(T, 0, sizeof(*T));
+  T->p0 = t->p0;
+  for (k = 0; k < n; k++) {
+    T->ak0[k] = (model == ORC_EULER) ? 1. : t->alpha[k];
+    T->rhok0[k] = orc_eos_rho_pt(&eos[k], eps, t->p0, t->T0);
+  }
+  if (model == ORC_EULER) {
+    T->Yk0[0] = 1.;
+    T->H0 = orc_eos_energy(&eos[0], eps, T->rhok0[0], T->p0) + T->p0 / omax(T->rhok0[0], eps);
+    return;
+  }
+  T->rho0 = 0.;
+  for (k = 0; k < n; k++) T->rho0 += T->ak0[k] * T->rhok0[k];
+  T->H0 = 0.;
+  for (k = 0; k < n; k++) {
+    T->Yk0[k] = T->ak0[k] * T->rhok0[k] / T->rho0;
+    T->H0 += T->Yk0[k] * (orc_eos_energy(&eos[k], eps, T->rhok0[k], T->p0) + T->p0 / omax(T->rhok0[k], eps));
+  }
+}
+
+/* ModEuler::solveRiemannInletTank, ModEuler.cpp:413-545, in the face frame (normal out of the
+   domain).  Returns the Newton iterations of the tank branch (0 outside it), -1 when 50 are spent. */
+static int tank_euler(const orc_eos* q, double eps, const tank_t* T, const cst* Lc, double dxL, double* dtMax,
+                      flux_t* F)
+{
+  double uL = Lc->u[0], vL = Lc->u[1], wL = Lc->u[2], pL = Lc->p[0], rhoL = Lc->r[0], cL = Lc->c[0];
+  double zL = rhoL * cL, p0 = T->p0;
+  double uStar, rhoStar, pStar, eStar, vStar, wStar, v, vmv0, mL, sL, u;
+  int iteration = 0;
+  pStar = p0;
+  v = 1. / orc_eos_rho_isentropic(q, eps, pL, rhoL, pStar, NULL);
+  vmv0 = v - 1. / rhoL;
+  if (fabs(vmv0) > 1e-10) mL = sqrt((pL - p0) / vmv0);
+  else mL = zL;
+  sL = uL - mL / rhoL;
+  dt_update(dtMax, sL, dxL);
+  u = uL + mL * vmv0;
+  if (sL >= 0.) {
+    uStar = uL; pStar = pL; rhoStar = rhoL; vStar = vL; wStar = wL;
+  } else if (u >= -1e-3) {
+    uStar = u; pStar = p0; rhoStar = 1. / v; vStar = vL; wStar = wL;
+  } else {
+    double H0 = T->H0, p = 0.5 * p0, f = 0., df = 1., dv, h, dh, drho;
+    double uStarR = 0., duStarR = 0., uStarL = 0., duStarL = 0., dmL;
+    do {
+      p -= f / df;
+      iteration++;
+      if (iteration > 50) return -1;
+      p = orc_eos_clamp_p(q, p);
+      if (p > p0) p = p0 - 1e-6;
+      rhoStar = orc_eos_rho_isentropic(q, eps, p0, T->rhok0[0], p, NULL);
+      h = orc_eos_h_isentropic(q, eps, p0, T->rhok0[0], p, &dh);
+      uStarR = -sqrt(2. * (H0 - h));
+      duStarR = -dh / uStarR;
+      v = 1.0 / orc_eos_rho_isentropic(q, eps, pL, rhoL, p, &drho);
+      dv = -v * v * drho;
+      vmv0 = v - 1. / rhoL;
+      if (fabs(vmv0) > 1e-10) {
+        mL = sqrt((pL - p) / vmv0);
+        dmL = 0.5 * (-vmv0 + (p - pL) * dv) / (vmv0 * vmv0) / mL;
+      } else {
+        mL = zL;
+        dmL = 0.;
+      }
+      sL = uL - mL / rhoL;
+      dt_update(dtMax, sL, dxL);
+      uStarL = uL + mL * vmv0;
+      duStarL = dmL * vmv0 + mL * dv;
+      f = uStarR - uStarL;
+      df = duStarR - duStarL;
+    } while (fabs(f) > 1e-3);
+    pStar = p;
+    uStar = 0.5 * (uStarL + uStarR);
+    vStar = 0.;
+    wStar = 0.;
+  }
+  eStar = orc_eos_energy(q, eps, rhoStar, pStar);
+  memset(F, 0, sizeof(*F));
+  F->f.mass[0] = rhoStar * uStar;
+  F->f.mom[0] = rhoStar * uStar * uStar + pStar;
+  F->f.mom[1] = rhoStar * uStar * vStar;
+  F->f.mom[2] = rhoStar * uStar * wStar;
+  F->f.energM = (rhoStar * (eStar + 0.5 * (uStar * uStar + vStar * vStar + wStar * wStar)) + pStar) * uStar;
+  F->sM = uStar;
+  F->uStar = uStar;
+  return iteration;
+}
+
+/* ModUEq::solveRiemannInletTank, ModUEq.cpp:496-673 (shared by PRESSUREVELOCITYEQ), in the face
+   frame.  cL is the mixture's Wood sound speed (MixUEq.cpp:152-159).  Returns as tank_euler. */
+static int tank_ueq(int n, const orc_eos* eos, double eps, const tank_t* T, const cst* Lc, double dxL,
+                    double* dtMax, flux_t* F)
+{
+  double rks[MAXPH], Yks[MAXPH];
+  double uL = Lc->u[0], pL = Lc->pm, rhoL = Lc->rho, uyL = Lc->u[1], uzL = Lc->u[2];
+  double cL = 0., zL, sL, sM, vmv0, mL, p0 = T->p0;
+  double pStar, uStar = 0., rhoStar = 0., vStar, uyStar, uzStar, EStar, ek, akStar;
+  int k, iteration = 0;
+  for (k = 0; k < n; k++) cL += Lc->a[k] / omax((Lc->r[k] * Lc->c[k] * Lc->c[k]), eps);
+  cL = 1. / sqrt(rhoL * cL);
+  zL = rhoL * cL;
+  /* 1) left wave with pStar = p0 (Hugoniot densities) */
+  pStar = p0;
+  vStar = 0.;
+  for (k = 0; k < n; k++) {
+    rks[k] = orc_eos_rho_hugoniot(&eos[k], eps, Lc->p[k], Lc->r[k], pStar);
+    vStar += Lc->a[k] * Lc->r[k] / rhoL / omax(rks[k], eps);
+  }
+  vmv0 = vStar - 1. / rhoL;
+  if (fabs(vmv0) > 1e-10) mL = sqrt((pL - pStar) / vmv0);
+  else mL = zL;
+  sL = uL - mL / rhoL;
+  dt_update(dtMax, sL, dxL);
+  sM = uL + mL * vmv0;
+  /* 2) outflow */
+  if (sL >= 0.) {
+    uStar = uL;
+    pStar = pL;
+    for (k = 0; k < n; k++) {
+      rks[k] = Lc->r[k];
+      Yks[k] = Lc->a[k] * Lc->r[k] / rhoL;
+    }
+    rhoStar = rhoL;
+    uyStar = uyL;
+    uzStar = uzL;
+  } else if (sM >= -1e-3) {
+    uStar = sM;
+    pStar = p0;
+    for (k = 0; k < n; k++) Yks[k] = Lc->a[k] * Lc->r[k] / rhoL;
+    rhoStar = 1. / vStar;
+    uyStar = uyL;
+    uzStar = uzL;
+  } else {
+    /* 3) tank inflow: Newton on p, tank isentrope and H = H0 on the right, isentropes on the left */
+    double p = 0.5 * p0, f = 0., df = 1., hk, dhk, rhok, drhok, dmL, YkL;
+    double uStarR = 0., duStarR = 0., uStarL = 0., duStarL = 0., vStarL, dvStarL;
+    do {
+      p -= f / df;
+      iteration++;
+      for (k = 0; k < n; k++) p = orc_eos_clamp_p(&eos[k], p);
+      if (p > p0) p = p0 - 1e-6;
+      if (iteration > 50) return -1;
+      uStarR = T->H0;
+      duStarR = 0.;
+      for (k = 0; k < n; k++) {
+        rks[k] = orc_eos_rho_isentropic(&eos[k], eps, p0, T->rhok0[k], p, NULL);
+        hk = orc_eos_h_isentropic(&eos[k], eps, p0, T->rhok0[k], p, &dhk);
+        uStarR -= T->Yk0[k] * hk;
+        duStarR -= T->Yk0[k] * dhk;
+      }
+      uStarR = -sqrt(2. * uStarR);
+      duStarR = duStarR / uStarR;
+      vStarL = 0.;
+      dvStarL = 0.;
+      for (k = 0; k < n; k++) {
+        rhok = orc_eos_rho_isentropic(&eos[k], eps, Lc->p[k], Lc->r[k], p, &drhok);
+        YkL = Lc->a[k] * Lc->r[k] / rhoL;
+        vStarL += YkL / omax(rhok, eps);
+        dvStarL -= YkL / omax((rhok * rhok), eps) * drhok;
+      }
+      vmv0 = vStarL - 1. / rhoL;
+      if (fabs(vmv0) > 1e-10) {
+        mL = sqrt((pL - p) / vmv0);
+        dmL = 0.5 * (-vmv0 + (p - pL) * dvStarL) / (vmv0 * vmv0) / mL;
+      } else {
+        mL = zL;
+        dmL = 0.;
+      }
+      sL = uL - mL / rhoL;
+      dt_update(dtMax, sL, dxL);
+      uStarL = uL + mL * vmv0;
+      duStarL = dmL * vmv0 + mL * dvStarL;
+      f = uStarR - uStarL;
+      df = duStarR - duStarL;
+    } while (fabs(f) > 1e-2);
+    pStar = p;
+    uStar = 0.5 * (uStarL + uStarR);
+    rhoStar = 0.;
+    for (k = 0; k < n; k++) {
+      Yks[k] = T->Yk0[k];
+      rhoStar += Yks[k] / omax(rks[k], eps);
+    }
+    rhoStar = 1. / rhoStar;
+    uyStar = 0.;
+    uzStar = 0.;
+    sM = uStar;
+  }
+  /* 4) flux (ModUEq.cpp:643-665) */
+  memset(F, 0, sizeof(*F));
+  EStar = 0.5 * (uStar * uStar + uyStar * uyStar + uzStar * uzStar);
+  for (k = 0; k < n; k++) {
+    ek = orc_eos_energy(&eos[k], eps, rks[k], pStar);
+    EStar += Yks[k] * ek;
+    akStar = Yks[k] * rhoStar / omax(rks[k], eps);
+    F->f.alpha[k] = akStar * sM;
+    F->f.mass[k] = akStar * uStar * rks[k];
+    F->f.energ[k] = F->f.mass[k] * ek;
+  }
+  F->f.mom[0] = rhoStar * uStar * uStar + pStar;
+  F->f.mom[1] = rhoStar * uStar * uyStar;
+  F->f.mom[2] = rhoStar * uStar * uzStar;
+  F->f.energM = (rhoStar * EStar + pStar) * uStar;
+  F->sM = sM;
+  F->uStar = uStar;
+  return iteration;
 }
 
 static void riemann_intern(const orc_ctx* c, const cst* L, const cst* R, double* dtMax, flux_t* F)
@@ -988,6 +1242,8 @@ orc_ctx* orc_create(const orc_config* cfg)
   c->face_off[1] = c->nface_int[0];
   c->face_off[2] = c->nface_int[0] + c->nface_int[1];
   c->slopes = (slope_t*)calloc((size_t)(c->nface_int[0] + c->nface_int[1] + c->nface_int[2] + 1), sizeof(slope_t));
+  for (a = 0; a < 6; a++)
+    if (cfg->bc[a] == ORC_BC_INLET_TANK) tank_setup(cfg->model, c->n, c->cfg.eos, c->eps, &cfg->tank[a], &c->tk[a]);
   return c;
 }
 
@@ -1147,6 +1403,14 @@ static void boundary_face(orc_ctx* c, long long L, int fr, int bc, const cst* sr
   if (bc == ORC_BC_WALL) {
     if (c->cfg.model == ORC_EULER) wall_euler(&b, c->lcfl, dtMax, &F);
     else wall_ueq(&b, c->lcfl, dtMax, &F);
+  } else if (bc == ORC_BC_INLET_TANK) {
+    const tank_t* T = &c->tk[fr - F_XM];
+    const int it = (c->cfg.model == ORC_EULER) ? tank_euler(EOSK(c, 0), c->eps, T, &b, c->lcfl, dtMax, &F)
+                                               : tank_ueq(c->n, c->cfg.eos, c->eps, T, &b, c->lcfl, dtMax, &F);
+    if (it < 0) {
+      c->tank_fail++;
+      return;
+    }
   } else {
     riemann_intern(c, &b, &b, dtMax, &F);
   }
@@ -1574,6 +1838,7 @@ int orc_step(orc_ctx* c, double dt, double* dtmax_out, orc_stats* st)
 {
   long long i;
   double dtMax = 1.e10;
+  c->tank_fail = 0;
   memset(c->acc, 0, sizeof(cons_t) * (size_t)c->ncell);
   if (c->cfg.order == 1) {
     /* Run::solveHyperbolic (Run.cpp:696-716) */
@@ -1607,7 +1872,8 @@ int orc_step(orc_ctx* c, double dt, double* dtmax_out, orc_stats* st)
   /* Run::advancingProcedure step 4 (Run.cpp:593) */
   for (i = 0; i < c->ncell; i++) relax_and_correct(c, &c->W[i], dt, st);
   if (dtmax_out) *dtmax_out = dtMax;
-  return 0;
+  /* Errors::errorMessage("solveRiemannInletTank not converged ...") ends the reference's run */
+  return c->tank_fail ? -2 : 0;
 }
 
 int orc_run(orc_ctx* c, int max_iter, double t_end, double* t_out, double* dt_next_out, int* iter_out,
@@ -1617,7 +1883,7 @@ int orc_run(orc_ctx* c, int max_iter, double t_end, double* t_out, double* dt_ne
   double dt = 1.e-15, t = 0., dtMax;
   int it = 0, done = 0;
   while (!done) {
-    orc_step(c, dt, &dtMax, st);
+    if (orc_step(c, dt, &dtMax, st) != 0) return -2;
     t += dt;
     it++;
     if (max_iter > 0) {
@@ -1686,4 +1952,41 @@ void orc_hllc_ueq(int n, const orc_eos* eos, double eps, const double* Lv, const
   out[o++] = F.sM;
   out[o++] = F.uStar;
   if (dtmax) *dtmax = dtm;
+}
+
+int orc_tank_flux(int model, int n, const orc_eos* eos, double eps, const orc_tank* t, const double* a,
+                  const double* r, const double* p, const double* u, double lcfl, double* out, double* dtmax)
+{
+  orc_ctx c;
+  cst L;
+  flux_t F;
+  tank_t T;
+  int k, o = 0, it;
+  double dtm = 1.e10;
+  memset(&c, 0, sizeof(c));
+  c.cfg.model = model;
+  c.n = n;
+  c.eps = eps;
+  for (k = 0; k < n; k++) c.cfg.eos[k] = eos[k];
+  memset(&L, 0, sizeof(L));
+  for (k = 0; k < n; k++) {
+    L.a[k] = (model == ORC_EULER) ? 1. : a[k];
+    L.r[k] = r[k];
+    L.p[k] = p[k];
+  }
+  L.u[0] = u[0]; L.u[1] = u[1]; L.u[2] = u[2];
+  fulfill(&c, &L);
+  tank_setup(model, n, eos, eps, t, &T);
+  it = (model == ORC_EULER) ? tank_euler(&eos[0], eps, &T, &L, lcfl, &dtm, &F)
+                            : tank_ueq(n, eos, eps, &T, &L, lcfl, &dtm, &F);
+  if (it < 0) return it;
+  for (k = 0; k < n; k++) out[o++] = F.f.alpha[k];
+  for (k = 0; k < n; k++) out[o++] = F.f.mass[k];
+  for (k = 0; k < n; k++) out[o++] = F.f.energ[k];
+  out[o++] = F.f.mom[0]; out[o++] = F.f.mom[1]; out[o++] = F.f.mom[2];
+  out[o++] = F.f.energM;
+  out[o++] = F.sM;
+  out[o++] = F.uStar;
+  if (dtmax) *dtmax = dtm;
+  return it;
 }

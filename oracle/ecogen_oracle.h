@@ -25,7 +25,7 @@ enum { ORC_EULER = 0, ORC_PUEQ = 1, ORC_UEQ = 2 };
 enum { ORC_EOS_IG = 0, ORC_EOS_SG = 1 };
 enum { ORC_LIM_NONE = 0, ORC_LIM_MINMOD = 1, ORC_LIM_VANLEER = 2, ORC_LIM_VANALBADA = 3,
        ORC_LIM_SUPERBEE = 4, ORC_LIM_MC = 5, ORC_LIM_THINC = 6 };
-enum { ORC_BC_NONREFLECTING = 0, ORC_BC_WALL = 1 };
+enum { ORC_BC_NONREFLECTING = 0, ORC_BC_WALL = 1, ORC_BC_INLET_TANK = 2 };
 enum { ORC_DOM_ENTIRE = 0, ORC_DOM_HALFSPACE = 1, ORC_DOM_DISC = 2, ORC_DOM_SPHERE = 3,
        ORC_DOM_RECTANGLE = 4, ORC_DOM_CUBOID = 5 };
 
@@ -33,6 +33,12 @@ typedef struct {
   int type;            /* ORC_EOS_IG / ORC_EOS_SG */
   double gamma, pinf, cv, eref;
 } orc_eos;
+
+/* Tank of an inletTank side (BoundCondInletTank.cpp:37-165): p0, T0 and the volume fractions. */
+typedef struct {
+  double p0, T0;
+  double alpha[ORC_MAXPH];
+} orc_tank;
 
 typedef struct {
   int nx, ny, nz;
@@ -47,6 +53,7 @@ typedef struct {
   int bc[6];           /* X-, X+, Y-, Y+, Z-, Z+ */
   double cfl;
   double relax_rate;   /* > 0: finite-rate pressure relaxation (Euler solver) with this rate instead */
+  orc_tank tank[6];    /* state of each ORC_BC_INLET_TANK side */
 } orc_config;
 
 typedef struct {
@@ -77,7 +84,8 @@ int orc_fill(orc_ctx* c, const orc_domain* doms, int ndom);
 /* Persistent primitive set, flat SoA [nvar][ncell]. */
 int orc_get_state(const orc_ctx* c, double* out);
 int orc_set_state(orc_ctx* c, const double* in);
-/* One full step of size dt; returns dtMax (min over every Riemann call of the step). */
+/* One full step of size dt; returns dtMax (min over every Riemann call of the step).  Returns -2
+   when an inletTank Newton solve spent its 50 iterations (the reference stops with an error). */
 int orc_step(orc_ctx* c, double dt, double* dtmax, orc_stats* st);
 /* Reference time loop: dt0 = 1e-15, dt_{n+1} = cfl*dtMax_n; stops after max_iter or when t >= t_end. */
 int orc_run(orc_ctx* c, int max_iter, double t_end, double* t_out, double* dt_next_out, int* iter_out,
@@ -92,6 +100,17 @@ double orc_eos_p_isentropic(const orc_eos* q, double eps, double p0, double rho0
 double orc_eos_rho_pfinal(const orc_eos* q, double eps, double p0, double rho0, double pf, double* drhodp);
 double orc_eos_rho_hugoniot(const orc_eos* q, double eps, double p0, double rho0, double pf);
 double orc_eos_clamp_p(const orc_eos* q, double p);
+/* computeDensityIsentropic / computeEnthalpyIsentropic (with d/dp when the pointer is set) and
+   computeDensity(p, T) of EosSG / EosIG. */
+double orc_eos_rho_isentropic(const orc_eos* q, double eps, double p0, double rho0, double pf, double* drhodp);
+double orc_eos_h_isentropic(const orc_eos* q, double eps, double p0, double rho0, double pf, double* dhdp);
+double orc_eos_rho_pt(const orc_eos* q, double eps, double p, double T);
+/* Inlet-tank boundary flux (ModEuler.cpp:413-545 / ModUEq.cpp:496-673) of a cell with phases
+   a/r/p[n] (Euler: n = 1, a ignored) and face-frame velocity u[3] (normal out of the domain).
+   out as orc_hllc_ueq: alpha[n], mass[n], energ[n], mom[3], energMix, sM, uStar.  Returns the
+   Newton iterations of the tank-inflow branch (0 in the outflow branches), -1 after 50. */
+int orc_tank_flux(int model, int n, const orc_eos* eos, double eps, const orc_tank* t, const double* a,
+                  const double* r, const double* p, const double* u, double lcfl, double* out, double* dtmax);
 double orc_limiter(int kind, double s1, double s2);
 /* THINC volume fraction of a cell at the interface face (CellInterfaceO2Cartesian.cpp:133-149
    for side 0 = the face's left cell, :165-180 for side 1 = its right cell): a = the cell's

@@ -18,7 +18,7 @@ MAXPH = 4
 MODELS = {"euler": 0, "pueq": 1, "ueq": 2}
 EOS_TYPES = {"IG": 0, "SG": 1}
 LIMITERS = {"none": 0, "minmod": 1, "vanleer": 2, "vanalbada": 3, "superbee": 4, "mc": 5, "thinc": 6}
-BCS = {"nonreflecting": 0, "wall": 1, "symmetry": 1}
+BCS = {"nonreflecting": 0, "wall": 1, "symmetry": 1, "inlettank": 2}
 DOMAINS = {"entire": 0, "halfspace": 1, "disc": 2, "sphere": 3, "rectangle": 4, "cuboid": 5}
 
 
@@ -27,12 +27,16 @@ class OrcEos(C.Structure):
                 ("eref", C.c_double)]
 
 
+class OrcTank(C.Structure):
+    _fields_ = [("p0", C.c_double), ("T0", C.c_double), ("alpha", C.c_double * MAXPH)]
+
+
 class OrcConfig(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
                 ("model", C.c_int), ("nph", C.c_int), ("eos", OrcEos * MAXPH), ("order", C.c_int),
                 ("lim", C.c_int * 4), ("alpha_null", C.c_int), ("relax_p", C.c_int), ("bc", C.c_int * 6),
-                ("cfl", C.c_double), ("relax_rate", C.c_double)]
+                ("cfl", C.c_double), ("relax_rate", C.c_double), ("tank", OrcTank * 6)]
 
 
 class OrcDomain(C.Structure):
@@ -88,6 +92,17 @@ def lib():
         L.orc_hllc_ueq.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, dp, dp, C.c_double, dp, dp]
         L.orc_relax_finite.restype = C.c_int
         L.orc_relax_finite.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, C.c_double, C.c_double, dp, dp, dp]
+        L.orc_eos_rho_isentropic.restype = C.c_double
+        L.orc_eos_rho_isentropic.argtypes = [C.POINTER(OrcEos), C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_eos_h_isentropic.restype = C.c_double
+        L.orc_eos_h_isentropic.argtypes = [C.POINTER(OrcEos), C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_eos_rho_pt.restype = C.c_double
+        L.orc_eos_rho_pt.argtypes = [C.POINTER(OrcEos), C.c_double, C.c_double, C.c_double]
+        L.orc_tank_flux.restype = C.c_int
+        L.orc_tank_flux.argtypes = [C.c_int, C.c_int, C.POINTER(OrcEos), C.c_double, C.POINTER(OrcTank), dp, dp, dp,
+                                    dp, C.c_double, dp, dp]
+        L.orc_step.restype = C.c_int
+        L.orc_run.restype = C.c_int
         L.orc_relax_cell.restype = C.c_int
         L.orc_relax_cell.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, C.c_int, dp, dp, dp, dp]
         _LIB = L
@@ -121,7 +136,34 @@ def make_config(case):
         c.bc[i] = BCS[b.lower()]
     c.cfl = case.get("cfl", 0.8)
     c.relax_rate = float(case.get("relax_rate", 0.0))
+    for side, t in case.get("tanks", {}).items():
+        c.tank[int(side)] = make_tank(t)
     return c
+
+
+def make_tank(t):
+    """Tank of an inletTank side: dict(p0=, T0=, alpha=[...]) (alpha omitted for Euler)."""
+    o = OrcTank()
+    o.p0, o.T0 = float(t["p0"]), float(t["T0"])
+    for k, a in enumerate(t.get("alpha", [1.0])):
+        o.alpha[k] = float(a)
+    return o
+
+
+def tank_flux(model, eos, tank, a, r, p, u, lcfl=1.0, alpha_null=False):
+    """Point-wise inlet-tank boundary flux in the face frame; returns (out, dtmax, iterations)."""
+    L = lib()
+    n = len(eos)
+    oe = (OrcEos * n)(*[make_eos(e) for e in eos])
+    eps = 1e-15 if (alpha_null and model != "euler") else 0.0
+    aa = np.array(list(a) if a is not None else [1.0] * n, dtype=np.float64)
+    rr, pp, uu = (np.array(v, dtype=np.float64) for v in (r, p, u))
+    out = np.zeros(3 * n + 6)
+    dtm = np.zeros(1)
+    t = make_tank(tank)
+    it = L.orc_tank_flux(MODELS[model], n, oe, eps, C.byref(t), _dp(aa), _dp(rr), _dp(pp), _dp(uu), float(lcfl),
+                         _dp(out), _dp(dtm))
+    return out, float(dtm[0]), int(it)
 
 
 def make_domains(case):
@@ -181,12 +223,15 @@ class Oracle:
 
     def step(self, dt):
         dtm = np.zeros(1)
-        self.L.orc_step(self.h, float(dt), _dp(dtm), C.byref(self.stats))
+        if self.L.orc_step(self.h, float(dt), _dp(dtm), C.byref(self.stats)) != 0:
+            raise RuntimeError("solveRiemannInletTank not converged")
         return float(dtm[0])
 
     def run(self, max_iter=0, t_end=0.0):
         t = np.zeros(1)
         dtn = np.zeros(1)
         it = C.c_int(0)
-        self.L.orc_run(self.h, int(max_iter), float(t_end), _dp(t), _dp(dtn), C.byref(it), C.byref(self.stats))
+        if self.L.orc_run(self.h, int(max_iter), float(t_end), _dp(t), _dp(dtn), C.byref(it),
+                          C.byref(self.stats)) != 0:
+            raise RuntimeError("solveRiemannInletTank not converged")
         return float(t[0]), float(dtn[0]), int(it.value)

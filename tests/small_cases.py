@@ -164,6 +164,51 @@ def finite_relax(case, rate):
     return c
 
 
+def tank_variant(case, tanks, suffix="_tank"):
+    """inletTank sides (BoundCondInletTank): `tanks` maps a side (0..5 = X-, X+, Y-, Y+,
+    Z-, Z+) to its tank dict(p0=, T0=, alpha=[...])."""
+    c = dict(case)
+    bc = list(c.get("bc", ("nonreflecting",) * 6))
+    for side in tanks:
+        bc[side] = "inletTank"
+    c["bc"] = tuple(bc)
+    c["tanks"] = {side: dict(t) for side, t in tanks.items()}
+    c["name"] = case["name"] + suffix
+    return c
+
+
+def euler_sod_tank(order=1):
+    """Tank inflow at X- (p0 above the left state) and subsonic outflow into a
+    low-pressure tank at X+."""
+    return tank_variant(euler_sod(order), {0: dict(p0=3.e5, T0=400.), 1: dict(p0=5.e3, T0=300.)})
+
+
+def euler_2d_tank(order=1):
+    """Tank inflow at X- into the 80 m/s, 4e5 Pa stream, and supersonic outflow
+    (a +y jet at 500 m/s) through a tank at Y+."""
+    c = euler_2d(order)
+    c["domains"] = c["domains"] + [dict(kind="halfspace", axis=1, origin=0.7, dir=1, rho=[1.0], p=[1.e5],
+                                        vel=[0., 500., 0.])]
+    return tank_variant(c, {0: dict(p0=6.e5, T0=500.), 3: dict(p0=5.e4, T0=300.)})
+
+
+def pueq_2d_tank(order=1):
+    """Water/air mixture flowing in from a 3e5 Pa tank at X+, subsonic outflow
+    into a 5e4 Pa tank at Y-."""
+    return tank_variant(pueq_2d(order), {1: dict(p0=3.e5, T0=176.5, alpha=[0.9, 0.1]),
+                                         2: dict(p0=5.e4, T0=176.5, alpha=[0.5, 0.5])})
+
+
+def ueq3_3d_tank(order=1):
+    """Three-phase tank inflow at X+ and at Z-."""
+    return tank_variant(ueq3_3d(order), {1: dict(p0=4.e5, T0=300., alpha=[0.98, 0.01, 0.01]),
+                                         4: dict(p0=2.e5, T0=250., alpha=[0.2, 0.3, 0.5])})
+
+
+def ueq2_3d_tank(order=2):
+    return tank_variant(ueq2_3d(order), {1: dict(p0=3.e5, T0=300., alpha=[0.7, 0.3])})
+
+
 def all_cases():
     cs = [pueq_3d_tiles(2), pueq_2d_xcd(1), pueq_2d_xcd(2), euler_sod(1), euler_sod(2), euler_2d(1), euler_2d(2), euler_2d(2, lim="superbee"), euler_3d(1),
           euler_3d(2), wall_variant(euler_2d(1)), wall_variant(euler_3d(1)),
@@ -179,5 +224,8 @@ def all_cases():
           thinc(ueq2_3d(2), ("vanleer", "minmod", "minmod", "thinc")), thinc(wall_variant(ueq_2d(2))),
           # finite-rate pressure relaxation: slow (sub-steps over the whole dt) and fast (closing Newton)
           finite_relax(ueq_2d(1), 1e-7), finite_relax(ueq_2d(2), 1e-4), finite_relax(ueq3_3d(1), 1.0),
-          finite_relax(ueq3_3d(2), 1e-4)]
+          finite_relax(ueq3_3d(2), 1e-4),
+          # inlet tanks (BoundCondInletTank): inflow, subsonic and supersonic outflow
+          euler_sod_tank(1), euler_sod_tank(2), euler_2d_tank(1), euler_2d_tank(2), pueq_2d_tank(1),
+          pueq_2d_tank(2), ueq3_3d_tank(1), ueq2_3d_tank(2)]
     return {c["name"]: c for c in cs}

@@ -32,11 +32,12 @@ def test_library_loads_and_exports_every_declared_symbol():
 
 
 @pytest.mark.parametrize("cname,pyclass", [("ecg_config", ea.EcgConfig), ("ecg_domain", ea.EcgDomain),
-                                           ("ecg_eos", ea.EcgEos), ("ecg_step_stats", ea.EcgStepStats)])
+                                           ("ecg_eos", ea.EcgEos), ("ecg_step_stats", ea.EcgStepStats),
+                                           ("ecg_tank", ea.EcgTank)])
 def test_struct_layouts_match_header_field_order(cname, pyclass):
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     blocks = dict((name, body) for body, name in re.findall(r"typedef struct \{([^{}]*)\} (\w+);", text))
-    fields = re.findall(r"([a-z_0-9]+)(?:\[[A-Z_0-9]+\])?\s*[;,]", blocks[cname])
+    fields = re.findall(r"([A-Za-z_][A-Za-z_0-9]*)(?:\[[A-Z_0-9]+\])?\s*[;,]", blocks[cname])
     assert fields == [f for f, _ in pyclass._fields_]
 
 
@@ -53,6 +54,15 @@ def _case(**kw):
     (dict(eos=[dict(type="IG", gamma=0.9)]), "invalid EOS"),
     (dict(order=2, limiters=("none",) * 4), "limiters"),
     (dict(precision="fp32_learned"), "relaxing multiphase model"),
+    (dict(bc=("inletTank",) + ("nonreflecting",) * 5, tanks={0: dict(p0=-1., T0=300.)}), "p0 and T0"),
+    (dict(bc=("inletTank",) + ("nonreflecting",) * 5, tanks={0: dict(p0=1.e5, T0=300.)}), "needs cv > 0"),
+    (dict(bc=("inletTank",) + ("nonreflecting",) * 5, tanks={0: dict(p0=1.e5, T0=300., alpha=[0.5, 0.4])},
+          model="pueq", eos=[dict(type="SG", gamma=4.4, pinf=6e8, cv=1000.), dict(type="IG", gamma=1.4, cv=800.)]),
+     "sum of alpha"),
+    (dict(bc=("inletTank",) + ("nonreflecting",) * 5, tanks={0: dict(p0=1.e5, T0=300., alpha=[0.5, 0.5])},
+          model="pueq", precision="fp32_learned",
+          eos=[dict(type="SG", gamma=4.4, pinf=6e8, cv=1000.), dict(type="IG", gamma=1.4, cv=800.)]),
+     "without inlet tanks"),
 ])
 def test_invalid_configurations_are_rejected(kw, msg):
     lib = ea.library()

@@ -157,3 +157,71 @@ def test_finite_rate_relaxation(tmp_path):
     _edit(p, 'solver="euler"', 'solver="LSODA"')
     with pytest.raises(EcgError, match="LSODA"):
         read_case(d)
+
+
+TANK_XM = '<boundCond name="CLXm" type="nonReflecting" number="1"/>'
+
+
+def _tank_xml(fluids, data='<dataInletTank p0="3.e5" T0="300."/>'):
+    inner = "".join(f'\n      <dataFluid EOS="{e}" {a}/>' for e, a in fluids)
+    return (f'<boundCond name="CLXm" type="inletTank" number="1">\n    {data}\n    <fluidsProp>{inner}\n'
+            f'    </fluidsProp>\n  </boundCond>')
+
+
+def test_inlet_tank_boundary(tmp_path):
+    """<boundCond type="inletTank"> (BoundCondInletTank.cpp:37-165): volume fractions
+    matched to the phases by EOS name and normalised by their sum; mass fractions
+    turned into volume fractions through the phase densities at (p0, T0)."""
+    write_eos(str(tmp_path))
+    d = os.path.join(str(tmp_path), "b")
+    write_kapila_bubble(d, n=16, second_order=True)
+    p = os.path.join(d, "initialConditions.xml")
+    base = open(p).read()
+    open(p, "w").write(base.replace(TANK_XM, _tank_xml([("IG_air.xml", 'alpha="0.3"'),
+                                                        ("SG_water.xml", 'alpha="0.7000001"')])))
+    c = read_case(d)
+    assert c["bc"][0] == "inletTank" and c["bc"][1] == "nonreflecting"
+    t = c["tanks"][0]
+    assert t["p0"] == 3.e5 and t["T0"] == 300.
+    np.testing.assert_allclose(t["alpha"], [0.7000001 / 1.0000001, 0.3 / 1.0000001], rtol=1e-15)
+    o = Oracle(c)
+    for _ in range(3):
+        o.step(1.e-7)
+    assert np.all(np.isfinite(o.get_state()))
+    # mass fractions
+    open(p, "w").write(base.replace(TANK_XM, _tank_xml([("SG_water.xml", 'massFrac="0.5"'),
+                                                        ("IG_air.xml", 'massFrac="0.5"')])))
+    c = read_case(d)
+    rw = (3.e5 + 6.e8) / ((4.4 - 1.) * 1000. * 300.)
+    ra = 3.e5 / ((1.4 - 1.) * 800. * 300.)
+    vw, va = 0.5 / rw, 0.5 / ra
+    np.testing.assert_allclose(c["tanks"][0]["alpha"], [vw / (vw + va), va / (vw + va)], rtol=1e-14)
+    # a single phase takes alpha 1 without <fluidsProp>
+    ds = os.path.join(str(tmp_path), "sod")
+    write_sod(ds, n=50)
+    ps = os.path.join(ds, "initialConditions.xml")
+    _edit(ps, TANK_XM, '<boundCond name="CLXm" type="inletTank" number="1"><dataInletTank p0="2.e5" T0="350."/>'
+                       '</boundCond>')
+    c = read_case(ds)
+    assert c["bc"][0] == "inletTank" and c["tanks"][0]["p0"] == 2.e5 and c["tanks"][0]["alpha"] == [1.0]
+
+
+@pytest.mark.parametrize("fluids,data,msg", [
+    ([("IG_air.xml", 'alpha="0.3"'), ("SG_water.xml", 'massFrac="0.7"')], None, "only one of"),
+    ([("IG_air.xml", 'alpha="1."')], None, "per phase"),
+    ([("IG_air.xml", 'alpha="0.5"'), ("SG_water.xml", 'alpha="0.7"')], None, "sum of alpha"),
+    ([("IG_air.xml", 'alpha="-0.5"'), ("SG_water.xml", 'alpha="1.5"')], None, r"in \[0,1\]"),
+    ([("IG_air.xml", 'alpha="0.3"'), ("SG_vapor.xml", 'alpha="0.7"')], None, "unknown"),
+    ([("IG_air.xml", ""), ("SG_water.xml", "")], None, "One of following"),
+    ([("IG_air.xml", 'alpha="0.3"'), ("SG_water.xml", 'alpha="0.7"')], '<dataInletTank p0="3.e5"/>', "T0"),
+    ([("IG_air.xml", 'alpha="0.3"'), ("SG_water.xml", 'alpha="0.7"')], "<other/>", "dataInletTank"),
+])
+def test_malformed_inlet_tanks_are_rejected(tmp_path, fluids, data, msg):
+    write_eos(str(tmp_path))
+    d = os.path.join(str(tmp_path), "b")
+    write_kapila_bubble(d, n=16)
+    p = os.path.join(d, "initialConditions.xml")
+    kw = {} if data is None else dict(data=data)
+    _edit(p, TANK_XM, _tank_xml(fluids, **kw))
+    with pytest.raises(EcgError, match=msg):
+        read_case(d)

@@ -196,3 +196,30 @@ def test_trained_warm_start_saves_newton_iterations():
     err = np.sum(np.abs(mix(s_warm) - mix(s_cold))) / np.sum(np.abs(mix(s_cold)))
     print(f"[learned] mixture density rel L1 learned vs cold FP32 form: {err:.2e}", flush=True)
     assert err <= 1e-5
+
+
+@pytest.mark.gpu
+def test_learned_second_order_kernels_with_an_inlet_tank():
+    """The learned form of a second-order case with an inlet tank runs the
+    full-featured kernel instantiation; with the network output at zero (the
+    linearised guess) it must track the cold FP32 form."""
+    from ecogen_amd import Mlp, Solver
+    from small_cases import pueq_2d_tank
+    print("[learned] second order with an inlet tank", flush=True)
+    case = pueq_2d_tank(2)
+    states, stats = [], []
+    for precision in ("fp32", "fp32_learned"):
+        c = dict(case)
+        c["precision"] = precision
+        g = Solver(c)
+        if precision == "fp32_learned":
+            m = Mlp(seed=1)
+            m.set_params(np.zeros(mo.N_PARAM, dtype=np.float32))
+            g.set_warm_start(m)
+        stats.append(g.run(max_iter=10))
+        states.append(g.get_state())
+    assert stats[1]["warm_calls"] > 0 and np.all(np.isfinite(states[1]))
+    mix = lambda s: s[0] * s[1] + s[3] * s[4]
+    err = np.sum(np.abs(mix(states[1]) - mix(states[0]))) / np.sum(np.abs(mix(states[0])))
+    print(f"[learned] tank case: mixture density rel L1 learned vs cold {err:.2e}", flush=True)
+    assert err <= 1e-5
