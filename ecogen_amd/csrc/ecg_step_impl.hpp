@@ -297,6 +297,10 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
   R* s_flux = lds + (O2 ? (T + 3) * LB * SZ::NS : 0);
   const int tid = threadIdx.x;
   const int lown = (own[B] - org[B]) + TB * (own[C] - org[C]);   // line of the own cell
+  // Work slots of steps A and B run along the axis for x (contiguous cells), and
+  // across the lines for y and z, whose consecutive lines are consecutive x: a
+  // wave's loads then walk along x in every pass.
+  constexpr bool ACROSS = (AX != 0);
   // Second-order walls: their slopes enter the neighbouring cells' limiters.
   const bool wall_m = O2 && LK == LK_FULL && P.bc[2 * AX] == BC_WALL;
   const bool wall_p = O2 && LK == LK_FULL && P.bc[2 * AX + 1] == BC_WALL;
@@ -307,11 +311,12 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
     // A. face slopes s_g, g = -2 .. T (interior faces only)
     if (O2) {
       for (int q = tid; q < (T + 3) * LB; q += 256) {
-        const int g = q % (T + 3) - 2;
-        const int l = l0 + q / (T + 3);
+        const int g = (ACROSS ? q / LB : q % (T + 3)) - 2;
+        const int lb = ACROSS ? q % LB : q / (T + 3);
+        const int l = l0 + lb;
         const int cb = org[B] + l % TB, cc = org[C] + l / TB;
         const int m = org[AX] + g;
-        R* o = s_slope + q * SZ::NS;
+        R* o = s_slope + (lb * (T + 3) + g + 2) * SZ::NS;
         if (cb < n3[B] && cc < n3[C] && m >= 0 && m <= na - 2) {
           const int64_t cell = (int64_t)m * sa + (int64_t)cb * st3[B] + (int64_t)cc * st3[C];
           Cell<N, R> cl, cr;
@@ -331,8 +336,8 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
 
     // B. interior face fluxes f = -1 .. T-1
     for (int q = tid; q < (T + 1) * LB; q += 256) {
-      const int f = q % (T + 1) - 1;
-      const int lb = q / (T + 1);
+      const int f = (ACROSS ? q / LB : q % (T + 1)) - 1;
+      const int lb = ACROSS ? q % LB : q / (T + 1);
       const int l = l0 + lb;
       const int cb = org[B] + l % TB, cc = org[C] + l / TB;
       const int m = org[AX] + f;
@@ -370,7 +375,7 @@ __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>&
         }
         Flux<N, R> F;
         face_flux<MODEL, N, FR, R>(P, bl, bl.E, br, br.E, dtm, F);
-        flux_to_lds<MODEL, N, D, R>(F, s_flux + q * SZ::NF);
+        flux_to_lds<MODEL, N, D, R>(F, s_flux + (lb * (T + 1) + f + 1) * SZ::NF);
       }
     }
     __syncthreads();
