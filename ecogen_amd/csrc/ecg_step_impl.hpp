@@ -50,8 +50,13 @@ template <int D, int A, int ES> struct TileAxis {
 };
 
 // LDS bytes one batch of an axis pass may stage (slopes + fluxes); larger
-// passes are split into batches of lines so that more blocks fit on a CU.
-constexpr int LDS_BATCH_BYTES = 32768;
+// passes are split into batches of lines so that the resident blocks fit in a
+// CU's 160 KB: five blocks of 32 KB, or for the multiphase FP64 kernels, whose
+// registers allow three blocks per CU, three of 52 KB (the 8x8 lines of the x
+// and y passes then run in one batch).
+template <int MODEL, int ES> constexpr int lds_batch_bytes() {
+  return (ES == 8 && MODEL != M_EULER) ? 53248 : 32768;
+}
 
 // Variables kept per face slope: phase (alpha, rho, p) x N (Euler: rho, p), velocity on the active axes.
 template <int MODEL, int N, int D> struct Sizes {
@@ -66,7 +71,7 @@ template <int MODEL, int N, int D> struct Sizes {
   // Batches of lines of axis A.
   template <int A, int ES> static constexpr int batches(bool o2) {
     int nb = 1;
-    while (nb < TileAxis<D, A, ES>::LINES && per_line<A, ES>(o2) * (TileAxis<D, A, ES>::LINES / nb) * ES > LDS_BATCH_BYTES)
+    while (nb < TileAxis<D, A, ES>::LINES && per_line<A, ES>(o2) * (TileAxis<D, A, ES>::LINES / nb) * ES > lds_batch_bytes<MODEL, ES>())
       nb *= 2;
     return nb;
   }
@@ -275,7 +280,7 @@ __device__ __forceinline__ void tile_origin(const Params& P, int& i0, int& j0, i
 }
 
 // One axis pass (steps A, B, C above) for axis AX with face frame FR, in
-// batches of lines when the staging would exceed LDS_BATCH_BYTES.
+// batches of lines when the staging would exceed lds_batch_bytes.
 template <int MODEL, int N, int D, bool O2, typename R, int LK, int AX, int FR>
 __device__ __forceinline__ void axis_pass(const Params& P, const StepLaunchT<R>& L, R* lds, const int org[3],
                                           const int own[3], bool own_ok, const R* an, const R* pn, R& dtm,
