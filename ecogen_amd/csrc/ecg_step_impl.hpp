@@ -624,10 +624,7 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
 
   // Learned warm start for the block's cells that relax from distinct phase pressures.
   float y = 0.f;
-  if constexpr (LEARN && MODEL != M_EULER) {
-    __syncthreads();   // the tile's LDS is free from here on
-    y = mlp_block_infer(L.mlp, spread_ok, feat, lds_raw, s_mlp);
-  }
+  if constexpr (LEARN && MODEL != M_EULER) y = mlp_block_infer(L.mlp, spread_ok, feat, lds_raw, s_mlp);
 
   int it = 0;
   if (own_ok && MODEL != M_EULER && P.relax) {

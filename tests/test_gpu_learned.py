@@ -55,7 +55,7 @@ def test_capture_matches_oracle_relaxation(model, eos, alpha, rho, p):
     rec = smp.download()
     assert rec.shape[0] == case["nx"]
     x, pc, sp, tl, fl, sg = mo.features(alpha, p, eos)
-    assert abs(fl) > 1e-11
+    assert abs(fl) > 1e-6          # above the library's warm-start gate
     L = po.lib()
     n = len(eos)
     oe = (po.OrcEos * n)(*[po.make_eos(e) for e in eos])
