@@ -727,8 +727,10 @@ __device__ __noinline__ int tank_flux(const Params& P, const TankDev& T, const C
 
 // ---------------------------------------------------------------------------
 // Accumulation of face contributions into the cell's m_cons buffer.
-// FP64: plain double sums in the reference order.  FP32: float-float pairs
-// (TwoSum of every contribution) so the sum keeps ~48 significant bits.
+// FP64: plain double sums in the reference order.  FP32: Kahan-compensated
+// float sums (the running compensation carries the bits each addition drops;
+// four operations per contribution against six for a TwoSum pair, with the
+// same errors on the fourteen or so contributions a cell receives).
 // ---------------------------------------------------------------------------
 template <typename R> struct Acc;
 template <> struct Acc<double> {
@@ -738,16 +740,15 @@ template <> struct Acc<double> {
   __device__ __forceinline__ double value() const { return v; }
 };
 template <> struct Acc<float> {
-  float hi = 0.f, lo = 0.f;
+  float hi = 0.f, c = 0.f;   // Kahan: c holds the negated low part lost by hi
   __device__ __forceinline__ void add(float x) {
-    const float s = hi + x;
-    const float bb = s - hi;
-    const float err = (hi - (s - bb)) + (x - bb);
-    hi = s;
-    lo += err;
+    const float y = x - c;
+    const float t = hi + y;
+    c = (t - hi) - y;
+    hi = t;
   }
   __device__ __forceinline__ void sub(float x) { add(-x); }
-  __device__ __forceinline__ double value() const { return (double)hi + (double)lo; }
+  __device__ __forceinline__ double value() const { return (double)hi - (double)c; }
 };
 
 template <int N, typename R>
