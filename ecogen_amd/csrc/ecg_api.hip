@@ -173,6 +173,7 @@ struct ecg_ctx {
   int esize = 8;                               // bytes per state element (8: FP64 form, 4: FP32 forms)
   int cur = 0;                                 // W[cur] holds the current state
   double* tables = nullptr;
+  double* tank_buf = nullptr;                  // flux records of the inlet-tank faces (k_tank)
   Clock* clk = nullptr;                        // device clock
   Counters* cnt = nullptr;
   Clock* hclk = nullptr;                       // pinned host mirror
@@ -317,12 +318,10 @@ static int check_config(const ecg_config* c) {
         return fail(ECG_E_UNSUPPORTED, "second order needs minmod/vanleer/vanalbada/superbee/mc/THINC limiters");
     }
   }
-  bool tank = false;
   for (int i = 0; i < 6; i++) {
     if (c->bc[i] != ECG_BC_NONREFLECTING && c->bc[i] != ECG_BC_WALL && c->bc[i] != ECG_BC_INLET_TANK)
       return fail(ECG_E_ARG, "unknown boundary");
     if (c->bc[i] != ECG_BC_INLET_TANK) continue;
-    tank = true;
     const ecg_tank& t = c->tank[i];
     if (!(t.p0 > 0.) || !std::isfinite(t.p0) || !(t.T0 > 0.) || !std::isfinite(t.T0))
       return fail(ECG_E_ARG, "inlet tank " + std::to_string(i) + ": p0 and T0 must be positive");
@@ -338,8 +337,6 @@ static int check_config(const ecg_config* c) {
       if (std::fabs(sum - 1.) > 1.e-6) return fail(ECG_E_ARG, "sum of alpha should be 1");
     }
   }
-  if (tank && c->order == 1 && c->precision == ECG_FP32_LEARNED)
-    return fail(ECG_E_UNSUPPORTED, "the learned warm start runs first-order cases without inlet tanks");
   if (!(c->cfl > 0)) return fail(ECG_E_ARG, "CFL must be positive");
   if (c->precision != ECG_FP64 && c->precision != ECG_FP32_RELAX64 && c->precision != ECG_FP32_LEARNED)
     return fail(ECG_E_ARG, "unknown precision form");
@@ -453,6 +450,12 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   for (int b = 0; b < nbuf && ok; b++) chk(hipMalloc(&c->W[b], bytes), "hipMalloc(state)");
   if (ok) chk(hipMalloc(&c->tables, sizeof(double) * tab.size()), "hipMalloc(tables)");
   if (ok) chk(hipMemcpy(c->tables, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice), "hipMemcpy(tables)");
+  // Inlet-tank face records, sides in order (the step kernels read them back by side and plane index).
+  int64_t ntank = 0;
+  for (int q = 0; q < 2 * c->D; q++)
+    if (P.bc[q] == BC_TANK) { P.tank_off[q] = ntank; ntank += tank_faces(P, q); }
+  if (ok && ntank > 0) chk(hipMalloc(&c->tank_buf, sizeof(double) * TANK_REC * ntank), "hipMalloc(tank records)");
+  P.tank_buf = c->tank_buf;
   if (ok) chk(hipMalloc(&c->clk, sizeof(Clock)), "hipMalloc(clock)");
   if (ok) chk(hipMalloc(&c->cnt, sizeof(Counters) * ATOMIC_SLOTS), "hipMalloc(counters)");
   if (ok) chk(hipHostMalloc(&c->hclk, sizeof(Clock), hipHostMallocDefault), "hipHostMalloc(clock)");
@@ -483,6 +486,7 @@ void ecg_destroy(ecg_ctx* c) {
   for (int b = 0; b < 3; b++)
     if (c->W[b]) hipFree(c->W[b]);
   if (c->tables) hipFree(c->tables);
+  if (c->tank_buf) hipFree(c->tank_buf);
   if (c->clk) hipFree(c->clk);
   if (c->cnt) hipFree(c->cnt);
   if (c->hclk) hipHostFree(c->hclk);

@@ -73,7 +73,18 @@ struct Params {
   int thinc;                // a volume-fraction limiter is THINC (two phases)
   double thinc_cosh, thinc_tanh;   // cosh(1.6), tanh(1.6) of the host's libm
   TankDev tank[6];          // sides whose bc is BC_TANK
+  double* tank_buf;         // flux records of the tank faces, TANK_REC doubles each (k_tank)
+  int64_t tank_off[6];      // first record of each tank side; faces of a side in plane order
 };
+
+// Flux record of one inlet-tank face (alpha, mass, energ per phase up to 4
+// phases, momentum in the global frame, mixture energy, sM, uStar).
+constexpr int TANK_REC = 3 * 4 + 6;
+// Faces of boundary side `side` (0..5 = x-, x+, y-, y+, z-, z+).
+inline int64_t tank_faces(const Params& P, int side) {
+  const int ax = side >> 1;
+  return (ax == 0) ? (int64_t)P.ny * P.nz : (ax == 1) ? (int64_t)P.nx * P.nz : (int64_t)P.nx * P.ny;
+}
 
 // Device clock: lagged time step kept on the GPU (Run.cpp:385-498).
 struct Clock {

@@ -413,52 +413,36 @@ __device__ __forceinline__ void boundary_face(const Params& P, const Cell<N, R>&
   sub_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
 }
 
-// Inlet-tank face of side `side` (BoundCondInletTank::solveRiemannBoundary,
-// BoundCondInletTank.cpp:209-212): the buffer cell is the stage state of the own
-// cell, as for every first-order boundary class; the tank solver runs in FP64.
-// Sets bad when the tank Newton did not converge (the face then adds nothing).
-template <int MODEL, int N, int D, int FR, typename R>
-__device__ __forceinline__ void tank_face(const Params& P, const Cell<N, R>& src, int side, R coef, const R* an,
-                                          const R* pn, R& dtm, ConsAcc<N, R>& acc, int& bad) {
-  Cell<N, double> b;
-#pragma unroll
-  for (int k = 0; k < N; k++) { b.a[k] = src.a[k]; b.r[k] = src.r[k]; b.p[k] = src.p[k]; }
-  b.u[0] = src.u[0]; b.u[1] = src.u[1]; b.u[2] = src.u[2];
-  derive<MODEL, N, double>(P, b, MODEL == M_EULER);
-  double lu[3];
-  to_local<FR, double>(b.u, lu);
-  double d64 = (double)dtm;
-  Flux<N, double> F64 = {};
-  if (tank_flux<MODEL, N>(P, P.tank[side], b, lu, d64, F64) < 0) { bad = 1; return; }
-  const double loc[3] = {F64.f.mom[0], F64.f.mom[1], F64.f.mom[2]};
-  to_global<FR, double>(loc, F64.f.mom);
-  dtm = dmin<R>(dtm, R(d64));
+// The step kernel's side of an inlet-tank face: the record k_tank wrote for it.
+template <int MODEL, int N, int D, int AX, int SIDE, typename R>
+__device__ __forceinline__ void tank_face(const Params& P, const int* own, R coef, const R* an, const R* pn,
+                                          ConsAcc<N, R>& acc) {
+  const int64_t f = (AX == 0) ? own[1] + (int64_t)P.ny * own[2]
+                  : (AX == 1) ? own[0] + (int64_t)P.nx * own[2]
+                              : own[0] + (int64_t)P.nx * own[1];
+  const R* rec = reinterpret_cast<const R*>(P.tank_buf) + (P.tank_off[2 * AX + SIDE] + f) * TANK_REC;
   Flux<N, R> F;
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    F.f.alpha[k] = R(F64.f.alpha[k]); F.f.mass[k] = R(F64.f.mass[k]); F.f.energ[k] = R(F64.f.energ[k]);
-  }
-  F.f.mom[0] = R(F64.f.mom[0]); F.f.mom[1] = R(F64.f.mom[1]); F.f.mom[2] = R(F64.f.mom[2]);
-  F.f.energM = R(F64.f.energM);
-  F.sM = R(F64.sM);
-  F.uStar = R(F64.uStar);
+  for (int k = 0; k < N; k++) { F.f.alpha[k] = R(rec[k]); F.f.mass[k] = R(rec[4 + k]); F.f.energ[k] = R(rec[8 + k]); }
+  F.f.mom[0] = R(rec[12]); F.f.mom[1] = R(rec[13]); F.f.mom[2] = R(rec[14]);
+  F.f.energM = R(rec[15]);
+  F.sM = R(rec[16]);
+  F.uStar = R(rec[17]);
   sub_flux<MODEL, N, D, R>(acc, F, coef, an, pn);
 }
 
-// Boundary face of axis AX, side SIDE (0: minus, 1: plus) of the own cell c.  A
-// second-order wall first extrapolates the cell's stage state to the face; an
-// inlet tank solves its own Riemann problem (both only in the LK_FULL kernels).
+// Boundary face of axis AX, side SIDE (0: minus, 1: plus) of the own cell c
+// (coordinates own).  A second-order wall first extrapolates the cell's stage
+// state to the face (LK_FULL kernels); an inlet tank's flux comes from k_tank.
 template <int MODEL, int N, int D, bool O2, int AX, int SIDE, int FR, typename R, int LK>
 __device__ __forceinline__ void boundary_face_at(const Params& P, const StepLaunchT<R>& L, const Cell<N, R>& src,
-                                                 int64_t c, const R* an, const R* pn, R& dtm, ConsAcc<N, R>& acc,
-                                                 int& tank_bad) {
+                                                 int64_t c, const int* own, const R* an, const R* pn, R& dtm,
+                                                 ConsAcc<N, R>& acc) {
   const int bc = P.bc[2 * AX + SIDE];
   const R coef = R(P.coef[AX]);
-  if constexpr (LK == LK_FULL) {
-    if (bc == BC_TANK) {
-      tank_face<MODEL, N, D, FR, R>(P, src, 2 * AX + SIDE, coef, an, pn, dtm, acc, tank_bad);
-      return;
-    }
+  if (bc == BC_TANK) {
+    tank_face<MODEL, N, D, AX, SIDE, R>(P, own, coef, an, pn, acc);
+    return;
   }
   if (!O2 || LK != LK_FULL || bc != BC_WALL) {
     boundary_face<MODEL, N, D, FR, R>(P, src, bc, coef, an, pn, dtm, acc);
@@ -496,6 +480,87 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// Inlet-tank faces (BoundCondInletTank::solveRiemannBoundary,
+// BoundCondInletTank.cpp:209-212), one thread each, ahead of the step kernel:
+// the buffer cell is the stage state of the boundary cell, as for every
+// first-order boundary class, and the tank solver runs in FP64.  Each face's
+// flux goes to its record of P.tank_buf, where the step kernel picks it up in
+// the reference's face order; the tank's wave speeds join the step's dtMax
+// through the counter slots.  A Newton that spends its 50 iterations counts in
+// tank_fail and leaves a zero flux.  Faces of a side are numbered in plane
+// order: (j, k) for x sides, (i, k) for y sides, (i, j) for z sides.
+template <int MODEL, int N, int FR, typename S>
+__device__ __forceinline__ int tank_record(const Params& P, const TankDev& T, const Cell<N, double>& b, double& dtm,
+                                           S* rec) {
+  double lu[3];
+  to_local<FR, double>(b.u, lu);
+  Flux<N, double> F = {};
+  const int it = tank_flux<MODEL, N>(P, T, b, lu, dtm, F);
+  if (it < 0) F = Flux<N, double>{};
+  const double loc[3] = {F.f.mom[0], F.f.mom[1], F.f.mom[2]};
+  to_global<FR, double>(loc, F.f.mom);
+#pragma unroll
+  for (int k = 0; k < N; k++) { rec[k] = S(F.f.alpha[k]); rec[4 + k] = S(F.f.mass[k]); rec[8 + k] = S(F.f.energ[k]); }
+  rec[12] = S(F.f.mom[0]); rec[13] = S(F.f.mom[1]); rec[14] = S(F.f.mom[2]);
+  rec[15] = S(F.f.energM); rec[16] = S(F.sM); rec[17] = S(F.uStar);
+  return it;
+}
+
+template <int MODEL, int N, int D, typename S>
+__global__ __launch_bounds__(256) void k_tank(Params P, const S* __restrict__ wsrc, const Clock* __restrict__ clk,
+                                              Counters* __restrict__ cnt, int64_t nfaces) {
+  __shared__ double s_dt[4];
+  __shared__ int s_bad[4];
+  if (clk->done) return;
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double dtm = 1.e10;
+  int bad = 0;
+  if (g < nfaces) {
+    // The sides' record ranges are disjoint and increasing: the last tank side starting at or before g owns it.
+    int side = 0;
+    for (int q = 0; q < 2 * D; q++)
+      if (P.bc[q] == BC_TANK && P.tank_off[q] <= g) side = q;
+    const int64_t f = g - P.tank_off[side];
+    const int ax = side >> 1, hi = side & 1;
+    const int64_t na = (ax == 0) ? P.ny : P.nx;                // first plane axis extent
+    const int64_t u = f % na, v = f / na;
+    int64_t i, j, k;
+    if (ax == 0) { i = hi ? P.nx - 1 : 0; j = u; k = v; }
+    else if (ax == 1) { i = u; j = hi ? P.ny - 1 : 0; k = v; }
+    else { i = u; j = v; k = hi ? P.nz - 1 : 0; }
+    const int64_t c = i + j * P.nx + k * (int64_t)P.nx * P.ny;
+    Cell<N, double> b;
+    load_prim<MODEL, N, D>(P, wsrc, c, b);
+    derive<MODEL, N, double>(P, b, MODEL == M_EULER);
+    // Records in the state's precision (the FP32 form's fluxes are float).
+    S* rec = reinterpret_cast<S*>(P.tank_buf) + g * TANK_REC;
+    const TankDev& T = P.tank[side];
+    int it = 0;
+    switch (side) {
+      case 0: it = tank_record<MODEL, N, FR_XM, S>(P, T, b, dtm, rec); break;
+      case 1: it = tank_record<MODEL, N, FR_XP, S>(P, T, b, dtm, rec); break;
+      case 2: it = tank_record<MODEL, N, FR_YM, S>(P, T, b, dtm, rec); break;
+      case 3: it = tank_record<MODEL, N, FR_YP, S>(P, T, b, dtm, rec); break;
+      case 4: it = tank_record<MODEL, N, FR_ZM, S>(P, T, b, dtm, rec); break;
+      default: it = tank_record<MODEL, N, FR_ZP, S>(P, T, b, dtm, rec); break;
+    }
+    bad = it < 0 ? 1 : 0;
+  }
+  const double m = wave_min(dtm);
+  const int nb = (int)wave_sum((unsigned long long)bad);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { s_dt[wv] = m; s_bad[wv] = nb; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = s_dt[0];
+    int bb = s_bad[0];
+    for (int q = 1; q < 4; q++) { mm = dmin<double>(mm, s_dt[q]); bb += s_bad[q]; }
+    Counters* slot = cnt + (blockIdx.x % ATOMIC_SLOTS);
+    atomicMin(&slot->dtmax_bits, (unsigned long long)__double_as_longlong(mm));
+    if (bb) atomicAdd(&slot->tank_fail, (unsigned long long)bb);
+  }
 }
 
 // Wave-aggregated append of relaxation samples (one atomic per wave).
@@ -573,22 +638,20 @@ __global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clo
     const bool on_bnd = (i == 0) || (i == P.nx - 1) || (D >= 2 && (j == 0 || j == P.ny - 1)) ||
                         (D >= 3 && (k == 0 || k == P.nz - 1));
     if (on_bnd) {
-      int tank_bad = 0;
       // Boundary faces see the stage's own cell: W^n (first order / predictor) or W* (corrector).
       Cell<N, R> src;
       load_prim<MODEL, N, D>(P, (O2 && L.stage == 2) ? L.wsrc : L.wn, c, src);
       derive<MODEL, N, R>(P, src, false);
-      if (i == 0) boundary_face_at<MODEL, N, D, O2, 0, 0, FR_XM, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
-      if (i == P.nx - 1) boundary_face_at<MODEL, N, D, O2, 0, 1, FR_XP, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
+      if (i == 0) boundary_face_at<MODEL, N, D, O2, 0, 0, FR_XM, R, LK>(P, L, src, c, own, an, pn, dtm, acc);
+      if (i == P.nx - 1) boundary_face_at<MODEL, N, D, O2, 0, 1, FR_XP, R, LK>(P, L, src, c, own, an, pn, dtm, acc);
       if (D >= 2) {
-        if (j == 0) boundary_face_at<MODEL, N, D, O2, 1, 0, FR_YM, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
-        if (j == P.ny - 1) boundary_face_at<MODEL, N, D, O2, 1, 1, FR_YP, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
+        if (j == 0) boundary_face_at<MODEL, N, D, O2, 1, 0, FR_YM, R, LK>(P, L, src, c, own, an, pn, dtm, acc);
+        if (j == P.ny - 1) boundary_face_at<MODEL, N, D, O2, 1, 1, FR_YP, R, LK>(P, L, src, c, own, an, pn, dtm, acc);
       }
       if (D >= 3) {
-        if (k == 0) boundary_face_at<MODEL, N, D, O2, 2, 0, FR_ZM, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
-        if (k == P.nz - 1) boundary_face_at<MODEL, N, D, O2, 2, 1, FR_ZP, R, LK>(P, L, src, c, an, pn, dtm, acc, tank_bad);
+        if (k == 0) boundary_face_at<MODEL, N, D, O2, 2, 0, FR_ZM, R, LK>(P, L, src, c, own, an, pn, dtm, acc);
+        if (k == P.nz - 1) boundary_face_at<MODEL, N, D, O2, 2, 1, FR_ZP, R, LK>(P, L, src, c, own, an, pn, dtm, acc);
       }
-      if (LK == LK_FULL && tank_bad) atomicAdd(&cnt[blockIdx.x % ATOMIC_SLOTS].tank_fail, 1ull);
     }
 
     // FP64 cell update from the time-n primitives.
@@ -777,13 +840,13 @@ bool has_tank(const Params& P) {
   return false;
 }
 
-// Second order: a case with a wall, an inlet tank, a THINC limiter or the
-// finite-rate relaxation runs the LK_FULL instantiation; otherwise a case whose
+// Second order: a case with a wall, a THINC limiter or the finite-rate
+// relaxation runs the LK_FULL instantiation; otherwise a case whose
 // four limiters are all minmod (the common choice) runs the instantiation with
 // the limiter fixed at compile time.
 template <int MODEL, int N, int D, typename R, bool LEARN>
 void launch_step_o2(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counters* cnt, hipStream_t s) {
-  bool full = P.thinc != 0 || P.relax_mu > 0. || has_tank<D>(P);
+  bool full = P.thinc != 0 || P.relax_mu > 0.;
   for (int a = 0; a < D; a++) full = full || P.bc[2 * a] == BC_WALL || P.bc[2 * a + 1] == BC_WALL;
   const bool all_minmod = P.lim[0] == LIM_MINMOD && P.lim[1] == LIM_MINMOD && P.lim[2] == LIM_MINMOD &&
                           P.lim[3] == LIM_MINMOD;
@@ -798,6 +861,14 @@ void launch_step_t(const Params& P, const StepLaunch& Lv, Clock* clk, Counters* 
   L.wn = (const R*)Lv.wn; L.wsrc = (const R*)Lv.wsrc; L.wout = (R*)Lv.wout; L.stage = Lv.stage;
   L.mlp = Lv.mlp;
   L.sink = Lv.sink;
+  // Inlet-tank faces first: their records must be in place when the step kernel gathers.
+  if (has_tank<D>(P)) {
+    int64_t nf = 0;
+    for (int q = 0; q < 2 * D; q++)
+      if (P.bc[q] == BC_TANK) nf += tank_faces(P, q);
+    hipLaunchKernelGGL((k_tank<MODEL, N, D, R>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, P, L.wsrc, clk,
+                       cnt, nf);
+  }
   // The learned warm start exists in the FP32 form of the relaxing models only.
   constexpr bool CAN_LEARN = (sizeof(R) == 4) && (MODEL != M_EULER);
   const bool learn = CAN_LEARN && Lv.mlp.frag != nullptr;
@@ -805,10 +876,8 @@ void launch_step_t(const Params& P, const StepLaunch& Lv, Clock* clk, Counters* 
     if constexpr (CAN_LEARN) {
       if (learn) { launch_step_v<MODEL, N, D, false, R, true, LK_GENERIC>(P, L, clk, cnt, s); return; }
     }
-    // First order: inlet tanks and the finite-rate relaxation live in LK_FULL
-    // (a learned first-order case has no tank side: the configuration check
-    // refuses it).
-    if (has_tank<D>(P) || (MODEL == M_UEQ && P.relax_mu > 0.)) {
+    // First order: the finite-rate relaxation lives in LK_FULL.
+    if (MODEL == M_UEQ && P.relax_mu > 0.) {
       launch_step_v<MODEL, N, D, false, R, false, LK_FULL>(P, L, clk, cnt, s);
       return;
     }

@@ -59,10 +59,6 @@ def _case(**kw):
     (dict(bc=("inletTank",) + ("nonreflecting",) * 5, tanks={0: dict(p0=1.e5, T0=300., alpha=[0.5, 0.4])},
           model="pueq", eos=[dict(type="SG", gamma=4.4, pinf=6e8, cv=1000.), dict(type="IG", gamma=1.4, cv=800.)]),
      "sum of alpha"),
-    (dict(bc=("inletTank",) + ("nonreflecting",) * 5, tanks={0: dict(p0=1.e5, T0=300., alpha=[0.5, 0.5])},
-          model="pueq", precision="fp32_learned",
-          eos=[dict(type="SG", gamma=4.4, pinf=6e8, cv=1000.), dict(type="IG", gamma=1.4, cv=800.)]),
-     "without inlet tanks"),
 ])
 def test_invalid_configurations_are_rejected(kw, msg):
     lib = ea.library()
