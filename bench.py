@@ -3,7 +3,8 @@
 
 Workload (default): the 3D two-phase pressure-non-equilibrium case
 (VELOCITYEQ, stiffened-gas water + ideal-gas air, 512^3 cells, 4096 air
-pockets with radii log-uniform over 2-10 cells, high-pressure slab drive,
+pockets with radii log-uniform over 2-10 cells, a 5e8 Pa inflow: a high-pressure
+slab fed through the x- side by an inlet tank,
 MUSCL minmod with the explicit midpoint predictor-corrector, infinite pressure
 relaxation).  Three forms run in the same process on the same case:
 
@@ -70,8 +71,8 @@ def traffic_from_profiles(key):
 
 
 WORKLOADS = {
-    "c5": "3D two-phase VELOCITYEQ water/air {n}^3, {extra} air pockets (r 2-10 cells), 5e8 Pa slab drive, "
-          "MUSCL minmod + explicit midpoint, infinite P-relaxation",
+    "c5": "3D two-phase VELOCITYEQ water/air {n}^3, {extra} air pockets (r 2-10 cells), 5e8 Pa inflow (a 5e8 Pa "
+          "slab fed by an inlet tank at x-), MUSCL minmod + explicit midpoint, infinite P-relaxation",
     "c4": "3D three-phase VELOCITYEQ water/epoxy/air {n}^3, {extra} inclusions, 1e9 Pa water shock, first order, "
           "infinite P-relaxation",
     "c3": "2D Kapila (PRESSUREVELOCITYEQ) water/air {n}^2, {extra} bubbles, 1e9 Pa water shock, MUSCL minmod + "
@@ -89,7 +90,7 @@ def make_case(workload, n):
     from ecogen_amd.case import kapila_bubbles_2d, ueq2_pockets_3d, ueq3_spheres_3d
     if workload == "c5":
         extra = 4096 if n == 512 else max(1, int(round(4096 * (n / 512.0) ** 3)))
-        return ueq2_pockets_3d(n=n, npockets=extra), extra
+        return ueq2_pockets_3d(n=n, npockets=extra, inflow_tank=True), extra
     if workload == "c4":
         extra = 64 if n == 256 else max(2, int(round(64 * (n / 256.0) ** 3)))
         return ueq3_spheres_3d(n=n, nsph=extra), extra

@@ -119,7 +119,7 @@ SIGNATURES = {
 }
 MLP_NPARAM = 5057
 SAMPLE_FLOATS = 16
-KERNEL_KINDS = ("step", "predict", "correct", "clock")
+KERNEL_KINDS = ("step", "predict", "correct", "clock", "tank")
 
 
 def library(path=None):
@@ -324,9 +324,10 @@ class Solver:
 
     def kernel_times(self):
         """{kind: (total ms, launches)} since profiling was switched on."""
-        ms = np.zeros(4)
-        n = (C.c_int64 * 4)()
-        _check(self.L.ecg_kernel_times(self.h, ms.ctypes.data_as(_DP), n, 4))
+        nk = len(KERNEL_KINDS)
+        ms = np.zeros(nk)
+        n = (C.c_int64 * nk)()
+        _check(self.L.ecg_kernel_times(self.h, ms.ctypes.data_as(_DP), n, nk))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNEL_KINDS)}
 
     @property

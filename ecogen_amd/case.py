@@ -96,8 +96,20 @@ def ueq3_spheres_3d(n=256, nsph=64, seed=SEED, p0=1.e5, p_shock=1.e9, x_shock=0.
                 alpha_null=False, relax_p=True, bc=("nonreflecting",) * 6, domains=doms)
 
 
-def ueq2_pockets_3d(n=512, npockets=4096, seed=SEED, p0=1.e5, p_drive=5.e8, x_drive=0.04, order=2):
-    """3D two-phase water/air pressure-non-equilibrium case: air pockets, high-pressure slab drive."""
+def drive_tank(eos, p_drive, rho1, u1, eps=EPS_PURE):
+    """Inlet tank (BoundCondInletTank) that feeds a stream of pressure p_drive, density rho1 and
+    speed u1: stagnation pressure p_drive + rho1 u1^2 / 2, and the temperature at which the EOS
+    gives the stream's isentrope density there."""
+    g, pinf, cv = eos["gamma"], eos.get("pinf", 0.0), eos["cv"]
+    p0 = p_drive + 0.5 * rho1 * u1 * u1
+    rho0 = rho1 * ((p0 + pinf) / (p_drive + pinf)) ** (1.0 / g)
+    return dict(p0=p0, T0=(p0 + pinf) / ((g - 1.0) * cv * rho0), alpha=[1. - eps, eps])
+
+
+def ueq2_pockets_3d(n=512, npockets=4096, seed=SEED, p0=1.e5, p_drive=5.e8, x_drive=0.04, order=2,
+                    inflow_tank=False):
+    """3D two-phase water/air pressure-non-equilibrium case: air pockets, high-pressure slab drive;
+    with inflow_tank the slab is fed through the x- side by an inlet tank (drive_tank)."""
     rng = np.random.default_rng(seed)
     water, air = _two_phase_states(p0)
     rho1, u1 = shocked_state(SG_WATER, p0, 1000.0, p_drive)
@@ -109,9 +121,14 @@ def ueq2_pockets_3d(n=512, npockets=4096, seed=SEED, p0=1.e5, p_drive=5.e8, x_dr
         doms.append(dict(kind="sphere", center=tuple(float(v) for v in c), radius=float(r), vel=[0., 0., 0.], **air))
     doms.append(dict(kind="halfspace", axis=0, origin=x_drive, dir=-1, vel=[u1, 0., 0.], **driven))
     lim = ("minmod",) * 4 if order == 2 else ("none",) * 4
-    return dict(name=f"ueq2_3d_pockets_{n}_o{order}", nx=n, ny=n, nz=n, lx=1.0, ly=1.0, lz=1.0, model="ueq",
+    case = dict(name=f"ueq2_3d_pockets_{n}_o{order}", nx=n, ny=n, nz=n, lx=1.0, ly=1.0, lz=1.0, model="ueq",
                 eos=[dict(SG_WATER), dict(IG_AIR)], order=order, limiters=lim, cfl=0.8, alpha_null=False,
                 relax_p=True, bc=("nonreflecting",) * 6, domains=doms)
+    if inflow_tank:
+        case["bc"] = ("inletTank",) + ("nonreflecting",) * 5
+        case["tanks"] = {0: drive_tank(SG_WATER, p_drive, rho1, u1)}
+        case["name"] += "_tank"
+    return case
 
 
 def variable_names(case):

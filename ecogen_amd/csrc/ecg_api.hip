@@ -174,6 +174,7 @@ struct ecg_ctx {
   int cur = 0;                                 // W[cur] holds the current state
   double* tables = nullptr;
   double* tank_buf = nullptr;                  // flux records of the inlet-tank faces (k_tank)
+  int64_t ntank = 0;                           // inlet-tank faces
   Clock* clk = nullptr;                        // device clock
   Counters* cnt = nullptr;
   Clock* hclk = nullptr;                       // pinned host mirror
@@ -186,8 +187,8 @@ struct ecg_ctx {
   bool profiling = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
-  double kt_ms[ECG_K_KINDS] = {0, 0, 0, 0};
-  int64_t kt_n[ECG_K_KINDS] = {0, 0, 0, 0};
+  double kt_ms[ECG_K_KINDS] = {};
+  int64_t kt_n[ECG_K_KINDS] = {};
   MlpDev mlp = {nullptr, nullptr};                // learned warm start (ecg_set_warm_start)
   SampleSink sink = {nullptr, nullptr, 0, 0, 1};  // relaxation sample capture (ecg_attach_samples)
 };
@@ -455,6 +456,7 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
   for (int q = 0; q < 2 * c->D; q++)
     if (P.bc[q] == BC_TANK) { P.tank_off[q] = ntank; ntank += tank_faces(P, q); }
   if (ok && ntank > 0) chk(hipMalloc(&c->tank_buf, sizeof(double) * TANK_REC * ntank), "hipMalloc(tank records)");
+  c->ntank = ntank;
   P.tank_buf = c->tank_buf;
   if (ok) chk(hipMalloc(&c->clk, sizeof(Clock)), "hipMalloc(clock)");
   if (ok) chk(hipMalloc(&c->cnt, sizeof(Counters) * ATOMIC_SLOTS), "hipMalloc(counters)");
@@ -607,13 +609,24 @@ static int enqueue_one(ecg_ctx* c) {
   StepLaunch L;
   L.mlp = c->mlp;
   L.sink = c->sink;
+  L.tank_only = 0;
+  // Inlet-tank face records of the stage, then the step kernel that reads them.
+  auto tank = [&] {
+    if (c->ntank == 0) return;
+    L.tank_only = 1;
+    timed(c, ECG_K_TANK, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream); });
+    L.tank_only = 0;
+  };
   if (c->cfg.order == 1) {
     L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[nxt]; L.stage = 0;
+    tank();
     timed(c, ECG_K_STEP, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream); });
   } else {
     L.wn = c->W[c->cur]; L.wsrc = c->W[c->cur]; L.wout = c->W[2]; L.stage = 1;
+    tank();
     timed(c, ECG_K_PREDICT, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream); });
     L.wn = c->W[c->cur]; L.wsrc = c->W[2]; L.wout = c->W[nxt]; L.stage = 2;
+    tank();
     timed(c, ECG_K_CORRECT, [&] { c->step(c->P, L, c->clk, c->cnt, c->stream); });
   }
   timed(c, ECG_K_CLOCK, [&] { launch_clock(c->clk, c->cnt, c->cfg.cfl, c->stream); });

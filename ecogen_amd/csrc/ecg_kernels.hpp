@@ -35,6 +35,7 @@ struct StepLaunch {
   int stage;            // 0: first order; 1: predictor; 2: corrector
   MlpDev mlp;           // learned warm start (frag == nullptr: cold start)
   SampleSink sink;      // relaxation sample capture (buf == nullptr: off)
+  int tank_only;        // 1: launch only k_tank for this stage's inlet-tank faces
 };
 
 template <typename R>

@@ -832,13 +832,6 @@ void launch_step_v(const Params& P, const StepLaunchT<R>& L, Clock* clk, Counter
   hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, P, L, clk, cnt);
 }
 
-// Whether a case has an inlet-tank side among its D axes.
-template <int D>
-bool has_tank(const Params& P) {
-  for (int a = 0; a < 2 * D; a++)
-    if (P.bc[a] == BC_TANK) return true;
-  return false;
-}
 
 // Second order: a case with a wall, a THINC limiter or the finite-rate
 // relaxation runs the LK_FULL instantiation; otherwise a case whose
@@ -861,13 +854,16 @@ void launch_step_t(const Params& P, const StepLaunch& Lv, Clock* clk, Counters* 
   L.wn = (const R*)Lv.wn; L.wsrc = (const R*)Lv.wsrc; L.wout = (R*)Lv.wout; L.stage = Lv.stage;
   L.mlp = Lv.mlp;
   L.sink = Lv.sink;
-  // Inlet-tank faces first: their records must be in place when the step kernel gathers.
-  if (has_tank<D>(P)) {
+  // Inlet-tank faces: launched on their own ahead of the step kernel, whose
+  // gather reads their records.
+  if (Lv.tank_only) {
     int64_t nf = 0;
     for (int q = 0; q < 2 * D; q++)
       if (P.bc[q] == BC_TANK) nf += tank_faces(P, q);
-    hipLaunchKernelGGL((k_tank<MODEL, N, D, R>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, P, L.wsrc, clk,
-                       cnt, nf);
+    if (nf > 0)
+      hipLaunchKernelGGL((k_tank<MODEL, N, D, R>), dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, P, L.wsrc,
+                         clk, cnt, nf);
+    return;
   }
   // The learned warm start exists in the FP32 form of the relaxing models only.
   constexpr bool CAN_LEARN = (sizeof(R) == 4) && (MODEL != M_EULER);

@@ -215,9 +215,10 @@ void* ecg_stream(ecg_ctx* ctx);
 
 /* Per-kernel timing with HIP events recorded around every launch on the
  * context's stream.  Kinds: 0 first-order step, 1 second-order predictor,
- * 2 second-order corrector, 3 clock update.  ecg_kernel_times accumulates the
- * elapsed milliseconds and launch counts since profiling was switched on. */
-enum { ECG_K_STEP = 0, ECG_K_PREDICT = 1, ECG_K_CORRECT = 2, ECG_K_CLOCK = 3, ECG_K_KINDS = 4 };
+ * 2 second-order corrector, 3 clock update, 4 inlet-tank faces (ahead of each
+ * step kernel).  ecg_kernel_times accumulates the elapsed milliseconds and
+ * launch counts since profiling was switched on. */
+enum { ECG_K_STEP = 0, ECG_K_PREDICT = 1, ECG_K_CORRECT = 2, ECG_K_CLOCK = 3, ECG_K_TANK = 4, ECG_K_KINDS = 5 };
 int ecg_set_profiling(ecg_ctx* ctx, int on);
 int ecg_kernel_times(ecg_ctx* ctx, double* ms_total, int64_t* launches, int nkinds);
 
