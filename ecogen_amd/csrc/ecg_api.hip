@@ -466,6 +466,10 @@ ecg_ctx* ecg_create(const ecg_config* cfg) {
     ecg_destroy(c);
     return nullptr;
   }
+  // The host copies mirror the zeroed device counters until the first read (pinned
+  // memory can come back from an earlier context with its old contents).
+  std::memset(c->hclk, 0, sizeof(Clock));
+  std::memset(c->hcnt, 0, sizeof(Counters) * ATOMIC_SLOTS);
   for (int a = 0; a < 3; a++) {
     P.dcc[a] = c->tables + off[a][0];
     P.dfl[a] = c->tables + off[a][1];
@@ -751,8 +755,10 @@ static int run_clock(ecg_ctx* c, const Clock& k, int64_t max_steps, ecg_step_sta
 int ecg_run(ecg_ctx* c, int64_t max_iter, double t_end, ecg_step_stats* st) {
   if (!c) return fail(ECG_E_ARG, "null context");
   if (max_iter <= 0 && !(t_end > 0)) return fail(ECG_E_ARG, "need max_iter > 0 or t_end > 0");
-  int rc = settle(c);
+  // A fresh read: run_clock compares the counters against their values before the run.
+  int rc = read_clock(c);
   if (rc != ECG_OK) return rc;
+  c->pending = 0;
   return run_clock(c, fresh_clock(1.e-15, max_iter, max_iter > 0 ? INFINITY : t_end), max_iter, st);
 }
 

@@ -209,6 +209,19 @@ def ueq2_3d_tank(order=2):
     return tank_variant(ueq2_3d(order), {1: dict(p0=3.e5, T0=300., alpha=[0.7, 0.3])})
 
 
+def tank_failure_case():
+    """A mixture at 1.1e8 Pa rushing out of a 1.4e4 Pa tank side at 2600 m/s: the tank
+    Newton (ModUEq.cpp:578-628) does not converge within its 50 iterations, where the
+    reference stops with an error."""
+    a = [0.7593086356253078, 0.24069136437469213]
+    return dict(name="tank_failure", nx=8, ny=1, nz=1, lx=1.0, ly=0.1, lz=0.1, model="ueq",
+                eos=[dict(SG_WATER), dict(IG_AIR)], order=1, limiters=("none",) * 4, cfl=0.8, relax_p=True,
+                bc=("inletTank",) + ("nonreflecting",) * 5,
+                tanks={0: dict(p0=13910.184292031761, T0=42.76872513378972, alpha=a)},
+                domains=[dict(kind="entire", alpha=a, rho=[1428.543048575648, 0.18663725254950703],
+                              p=[110096271.95814788] * 2, vel=[2607.606207829999, 0., 0.])])
+
+
 def all_cases():
     cs = [pueq_3d_tiles(2), pueq_2d_xcd(1), pueq_2d_xcd(2), euler_sod(1), euler_sod(2), euler_2d(1), euler_2d(2), euler_2d(2, lim="superbee"), euler_3d(1),
           euler_3d(2), wall_variant(euler_2d(1)), wall_variant(euler_3d(1)),

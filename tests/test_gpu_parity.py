@@ -86,3 +86,24 @@ def test_sod_time_loop_matches_reference_run():
     assert st["iteration"] == 433
     assert st["t"] == float("0.00050106045125640768")
     assert abs(st["dt_next"] - float("1.1542745158200889e-06")) <= 1e-15 * 1.2e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_non_converged_tank_newton_is_reported(precision):
+    """A tank Newton that spends its 50 iterations ends the step with ECG_E_STATE and the
+    reference's message (ModUEq.cpp:586), in a single step and in the device-clocked loop."""
+    from ecogen_amd import EcgError, Solver
+    from small_cases import tank_failure_case
+    print(f"[parity] tank failure {precision}", flush=True)
+    case = dict(tank_failure_case())
+    case["precision"] = precision
+    g = Solver(case)
+    with pytest.raises(EcgError, match="solveRiemannInletTank not converged"):
+        g.step(1.e-15)
+    # Fresh contexts after one that failed: each must count against its own counters
+    # (pinned host memory can come back from the earlier context with its contents).
+    for n in (1, 3):
+        g = Solver(case)
+        with pytest.raises(EcgError, match="solveRiemannInletTank not converged"):
+            g.run(max_iter=n)

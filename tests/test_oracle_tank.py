@@ -247,3 +247,15 @@ def test_tanks_at_both_ends_give_a_mirror_symmetric_flow(model):
             assert np.max(np.abs(a)) > 1.0          # the tanks drive an inflow from both ends
         scale = np.max(np.abs(s[v]))
         assert np.max(np.abs(a - b)) <= tol * scale, v
+
+
+def test_non_converged_tank_newton_stops_the_step():
+    """ModUEq.cpp:586: Errors::errorMessage("solveRiemannInletTank not converged")."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from small_cases import tank_failure_case
+    from oracle.pyoracle import Oracle
+    o = Oracle(tank_failure_case())
+    with pytest.raises(RuntimeError, match="solveRiemannInletTank not converged"):
+        o.step(1.e-15)
