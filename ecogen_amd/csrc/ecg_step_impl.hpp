@@ -587,8 +587,16 @@ __device__ __forceinline__ void capture_samples(const SampleSink& K, bool cap, c
   o[3] = make_float4(target, (float)iters, remainder, sigma);
 }
 
+// Waves per SIMD the step kernels are compiled for: three in FP64 (168 VGPRs);
+// five for the one- and two-phase FP32 kernels outside LK_FULL, whose 96-VGPR
+// budget costs them a few spilled values (the 512^3 FP32 step runs 3 % faster,
+// the learned one 5.6 %, than at the four their 105 VGPRs would otherwise
+// allow); three otherwise (three-phase FP32 kernels spill too much at five).
+template <typename R, int N, int LK>
+constexpr int step_waves() { return (sizeof(R) == 4 && N <= 2 && LK != LK_FULL) ? 5 : 3; }
+
 template <int MODEL, int N, int D, bool O2, typename R, bool LEARN, int LK>
-__global__ __launch_bounds__(256, 3) void k_step(Params P, StepLaunchT<R> L, Clock* __restrict__ clk,
+__global__ __launch_bounds__(256, (step_waves<R, N, LK>())) void k_step(Params P, StepLaunchT<R> L, Clock* __restrict__ clk,
                                               Counters* __restrict__ cnt) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   R* lds = reinterpret_cast<R*>(lds_raw);
