@@ -82,7 +82,7 @@ def _same_text(a, b, rel):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("which", ["sod_time", "pueq_o2", "ueq_o1_wall"])
+@pytest.mark.parametrize("which", ["sod_time", "pueq_o2", "ueq_o1_wall", "pueq_o1_tank"])
 def test_runner_matches_oracle(tmp_path, which):
     from ecogen_amd import read_case
     from oracle import writers as W
@@ -97,6 +97,20 @@ def test_runner_matches_oracle(tmp_path, which):
     elif which == "pueq_o2":
         write_kapila_bubble(d, n=40, iterations=15, second_order=True)
         opts = dict(fmt="GNU", precision=10)
+        names = KAPILA_EOS
+    elif which == "pueq_o1_tank":
+        # a water/air mixture flowing in from a 3e5 Pa tank at x+ (BoundCondInletTank)
+        write_kapila_bubble(d, n=40, iterations=15, second_order=False)
+        p = os.path.join(d, "initialConditions.xml")
+        s = open(p).read()
+        old = '<boundCond name="CLXp" type="nonReflecting" number="2"/>'
+        assert old in s
+        open(p, "w").write(s.replace(old, '<boundCond name="CLXp" type="inletTank" number="2">\n'
+                                          '    <dataInletTank p0="3.e5" T0="176.5"/>\n    <fluidsProp>\n'
+                                          '      <dataFluid EOS="SG_water.xml" alpha="0.9"/>\n'
+                                          '      <dataFluid EOS="IG_air.xml" alpha="0.1"/>\n'
+                                          '    </fluidsProp>\n  </boundCond>'))
+        opts = dict(fmt="XML", precision=10, global_quantities=("mass",))
         names = KAPILA_EOS
     else:
         write_kapila_bubble(d, n=40, iterations=15, second_order=False, model="VelocityEq")
