@@ -240,5 +240,10 @@ def all_cases():
           finite_relax(ueq3_3d(2), 1e-4),
           # inlet tanks (BoundCondInletTank): inflow, subsonic and supersonic outflow
           euler_sod_tank(1), euler_sod_tank(2), euler_2d_tank(1), euler_2d_tank(2), pueq_2d_tank(1),
-          pueq_2d_tank(2), ueq3_3d_tank(1), ueq2_3d_tank(2)]
+          pueq_2d_tank(2), ueq3_3d_tank(1), ueq2_3d_tank(2),
+          # tanks in the full-featured kernels: beside second-order walls, with the finite-rate relaxation
+          tank_variant(wall_variant(pueq_2d(2), ("wall", "nonreflecting", "wall", "wall", "nonreflecting",
+                                                 "nonreflecting")),
+                       {1: dict(p0=3.e5, T0=176.5, alpha=[0.9, 0.1])}),
+          tank_variant(finite_relax(ueq_2d(1), 1e-7), {1: dict(p0=3.e5, T0=300., alpha=[0.8, 0.2])})]
     return {c["name"]: c for c in cs}
