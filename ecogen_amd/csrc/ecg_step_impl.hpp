@@ -587,13 +587,15 @@ __device__ __forceinline__ void capture_samples(const SampleSink& K, bool cap, c
   o[3] = make_float4(target, (float)iters, remainder, sigma);
 }
 
-// Waves per SIMD the step kernels are compiled for: three in FP64 (168 VGPRs);
-// five for the one- and two-phase FP32 kernels outside LK_FULL, whose 96-VGPR
-// budget costs them a few spilled values (the 512^3 FP32 step runs 3 % faster,
-// the learned one 5.6 %, than at the four their 105 VGPRs would otherwise
-// allow); three otherwise (three-phase FP32 kernels spill too much at five).
+// Waves per SIMD the step kernels are compiled for: three in FP64 (168 VGPRs)
+// and in the FP32 LK_FULL kernels; five for the one- and two-phase FP32
+// kernels, whose 96-VGPR budget costs them a few spilled values (the 512^3 FP32
+// step runs 3 % faster, the learned one 5.6 %, than at the four their 105 VGPRs
+// would otherwise allow); four for the three-phase FP32 kernels (128 VGPRs
+// instead of 131: the 256^3 three-phase step runs 11 % faster than at three;
+// at five they spill too much).
 template <typename R, int N, int LK>
-constexpr int step_waves() { return (sizeof(R) == 4 && N <= 2 && LK != LK_FULL) ? 5 : 3; }
+constexpr int step_waves() { return (sizeof(R) == 4 && LK != LK_FULL) ? (N <= 2 ? 5 : 4) : 3; }
 
 template <int MODEL, int N, int D, bool O2, typename R, bool LEARN, int LK>
 __global__ __launch_bounds__(256, (step_waves<R, N, LK>())) void k_step(Params P, StepLaunchT<R> L, Clock* __restrict__ clk,
