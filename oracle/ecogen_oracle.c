@@ -8,6 +8,9 @@
  * source tree.
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1408,6 +1411,7 @@ static void boundary_face(orc_ctx* c, long long L, int fr, int bc, const cst* sr
     const int it = (c->cfg.model == ORC_EULER) ? tank_euler(EOSK(c, 0), c->eps, T, &b, c->lcfl, dtMax, &F)
                                                : tank_ueq(c->n, c->cfg.eos, c->eps, T, &b, c->lcfl, dtMax, &F);
     if (it < 0) {
+#pragma omp atomic
       c->tank_fail++;
       return;
     }
@@ -1431,43 +1435,60 @@ static void interior_face_o1(orc_ctx* c, long long L, long long R, int fr, doubl
   sub_flux(c, &c->acc[L], &F, coef, &c->W[L]);
 }
 
+/* Threads (OpenMP): every sweep below runs the faces of one line (or one boundary side) in the
+   serial order and gives different lines to different threads.  A face touches only cells of its
+   line, so each cell still receives its contributions in face-creation order and the results are
+   bit-identical for any thread count; dtMax is a min-reduction, exact in any order. */
 static void boundary_sweep(orc_ctx* c, const cst* src, double* dtMax)
 {
   int i, j, k;
   const orc_config* g = &c->cfg;
+  double dtm = *dtMax;
   if (g->nx != 1) {
+#pragma omp parallel for collapse(2) reduction(min : dtm) schedule(static)
     for (j = 0; j < g->ny; j++)
-      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, 0, j, k), F_XM, g->bc[0], src, c->coef[0], dtMax);
+      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, 0, j, k), F_XM, g->bc[0], src, c->coef[0], &dtm);
+#pragma omp parallel for collapse(2) reduction(min : dtm) schedule(static)
     for (j = 0; j < g->ny; j++)
-      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, g->nx - 1, j, k), F_XP, g->bc[1], src, c->coef[0], dtMax);
+      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, g->nx - 1, j, k), F_XP, g->bc[1], src, c->coef[0], &dtm);
   }
   if (g->ny != 1) {
+#pragma omp parallel for collapse(2) reduction(min : dtm) schedule(static)
     for (i = 0; i < g->nx; i++)
-      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, i, 0, k), F_YM, g->bc[2], src, c->coef[1], dtMax);
+      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, i, 0, k), F_YM, g->bc[2], src, c->coef[1], &dtm);
+#pragma omp parallel for collapse(2) reduction(min : dtm) schedule(static)
     for (i = 0; i < g->nx; i++)
-      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, i, g->ny - 1, k), F_YP, g->bc[3], src, c->coef[1], dtMax);
+      for (k = 0; k < g->nz; k++) boundary_face(c, cidx(c, i, g->ny - 1, k), F_YP, g->bc[3], src, c->coef[1], &dtm);
   }
   if (g->nz != 1) {
+#pragma omp parallel for collapse(2) reduction(min : dtm) schedule(static)
     for (i = 0; i < g->nx; i++)
-      for (j = 0; j < g->ny; j++) boundary_face(c, cidx(c, i, j, 0), F_ZM, g->bc[4], src, c->coef[2], dtMax);
+      for (j = 0; j < g->ny; j++) boundary_face(c, cidx(c, i, j, 0), F_ZM, g->bc[4], src, c->coef[2], &dtm);
+#pragma omp parallel for collapse(2) reduction(min : dtm) schedule(static)
     for (i = 0; i < g->nx; i++)
-      for (j = 0; j < g->ny; j++) boundary_face(c, cidx(c, i, j, g->nz - 1), F_ZP, g->bc[5], src, c->coef[2], dtMax);
+      for (j = 0; j < g->ny; j++) boundary_face(c, cidx(c, i, j, g->nz - 1), F_ZP, g->bc[5], src, c->coef[2], &dtm);
   }
+  *dtMax = dtm;
 }
 
 static void sweep_o1(orc_ctx* c, double* dtMax)
 {
   int i, j, k;
   const orc_config* g = &c->cfg;
-  for (i = 0; i < g->nx - 1; i++)
-    for (j = 0; j < g->ny; j++)
-      for (k = 0; k < g->nz; k++) interior_face_o1(c, cidx(c, i, j, k), cidx(c, i + 1, j, k), F_XI, c->coef[0], dtMax);
+  double dtm = *dtMax;
+#pragma omp parallel for collapse(2) private(i) reduction(min : dtm) schedule(static)
+  for (j = 0; j < g->ny; j++)
+    for (k = 0; k < g->nz; k++)
+      for (i = 0; i < g->nx - 1; i++) interior_face_o1(c, cidx(c, i, j, k), cidx(c, i + 1, j, k), F_XI, c->coef[0], &dtm);
+#pragma omp parallel for collapse(2) private(j) reduction(min : dtm) schedule(static)
   for (i = 0; i < g->nx; i++)
-    for (j = 0; j < g->ny - 1; j++)
-      for (k = 0; k < g->nz; k++) interior_face_o1(c, cidx(c, i, j, k), cidx(c, i, j + 1, k), F_YI, c->coef[1], dtMax);
+    for (k = 0; k < g->nz; k++)
+      for (j = 0; j < g->ny - 1; j++) interior_face_o1(c, cidx(c, i, j, k), cidx(c, i, j + 1, k), F_YI, c->coef[1], &dtm);
+#pragma omp parallel for collapse(2) private(k) reduction(min : dtm) schedule(static)
   for (i = 0; i < g->nx; i++)
     for (j = 0; j < g->ny; j++)
-      for (k = 0; k < g->nz - 1; k++) interior_face_o1(c, cidx(c, i, j, k), cidx(c, i, j, k + 1), F_ZI, c->coef[2], dtMax);
+      for (k = 0; k < g->nz - 1; k++) interior_face_o1(c, cidx(c, i, j, k), cidx(c, i, j, k + 1), F_ZI, c->coef[2], &dtm);
+  *dtMax = dtm;
   boundary_sweep(c, c->W, dtMax);
 }
 
@@ -1495,6 +1516,7 @@ static void compute_slopes(orc_ctx* c, const cst* S)
     n3[0] = g->nx; n3[1] = g->ny; n3[2] = g->nz;
     if (n3[axis] == 1) continue;
     n3[axis] -= 1;
+#pragma omp parallel for collapse(3) private(m) schedule(static)
     for (i = 0; i < n3[0]; i++)
       for (j = 0; j < n3[1]; j++)
         for (k = 0; k < n3[2]; k++) {
@@ -1813,15 +1835,21 @@ static void sweep_o2(orc_ctx* c, const cst* S, double* dtMax)
 {
   const orc_config* g = &c->cfg;
   int i, j, k;
-  for (i = 0; i < g->nx - 1; i++)
-    for (j = 0; j < g->ny; j++)
-      for (k = 0; k < g->nz; k++) interior_face_o2(c, 0, i, j, k, S, dtMax);
+  double dtm = *dtMax;
+  /* lines as in sweep_o1 */
+#pragma omp parallel for collapse(2) private(i) reduction(min : dtm) schedule(static)
+  for (j = 0; j < g->ny; j++)
+    for (k = 0; k < g->nz; k++)
+      for (i = 0; i < g->nx - 1; i++) interior_face_o2(c, 0, i, j, k, S, &dtm);
+#pragma omp parallel for collapse(2) private(j) reduction(min : dtm) schedule(static)
   for (i = 0; i < g->nx; i++)
-    for (j = 0; j < g->ny - 1; j++)
-      for (k = 0; k < g->nz; k++) interior_face_o2(c, 1, i, j, k, S, dtMax);
+    for (k = 0; k < g->nz; k++)
+      for (j = 0; j < g->ny - 1; j++) interior_face_o2(c, 1, i, j, k, S, &dtm);
+#pragma omp parallel for collapse(2) private(k) reduction(min : dtm) schedule(static)
   for (i = 0; i < g->nx; i++)
     for (j = 0; j < g->ny; j++)
-      for (k = 0; k < g->nz - 1; k++) interior_face_o2(c, 2, i, j, k, S, dtMax);
+      for (k = 0; k < g->nz - 1; k++) interior_face_o2(c, 2, i, j, k, S, &dtm);
+  *dtMax = dtm;
   boundary_sweep(c, S, dtMax);
 }
 
@@ -1836,13 +1864,14 @@ static void relax_and_correct(orc_ctx* c, cst* s, double dt, orc_stats* st)
 
 int orc_step(orc_ctx* c, double dt, double* dtmax_out, orc_stats* st)
 {
-  long long i;
+  long long i, rc = 0, ni = 0, nf = 0;   /* relaxation counters, summed over the threads */
   double dtMax = 1.e10;
   c->tank_fail = 0;
   memset(c->acc, 0, sizeof(cons_t) * (size_t)c->ncell);
   if (c->cfg.order == 1) {
     /* Run::solveHyperbolic (Run.cpp:696-716) */
     sweep_o1(c, &dtMax);
+#pragma omp parallel for schedule(static)
     for (i = 0; i < c->ncell; i++) {
       cst old = c->W[i];
       time_evolution(c, &c->acc[i], &old, dt);
@@ -1852,17 +1881,21 @@ int orc_step(orc_ctx* c, double dt, double* dtmax_out, orc_stats* st)
     /* Run::solveHyperbolicO2 (Run.cpp:627-692) with slopes from integrationProcedure (Run.cpp:526-533) */
     compute_slopes(c, c->W);
     sweep_o2(c, c->W, &dtMax);
+#pragma omp parallel for reduction(+ : rc, ni, nf) schedule(static)
     for (i = 0; i < c->ncell; i++) {
+      orc_stats ls = {0, 0, 0};
       /* CellO2::predictionOrdre2 (CellO2.cpp:111-135) */
       c->Ws[i] = c->W[i];
       time_evolution(c, &c->acc[i], &c->W[i], 0.5 * dt);
       build_prim(c, &c->acc[i], &c->Ws[i]);
-      if (c->cfg.relax_p) relax_cell(c, &c->Ws[i], 0.5 * dt, st);
+      if (c->cfg.relax_p) relax_cell(c, &c->Ws[i], 0.5 * dt, &ls);
       correct_and_fulfill(c, &c->Ws[i]);
+      rc += ls.relax_calls; ni += ls.newton_iters; nf += ls.newton_fail;
     }
     memset(c->acc, 0, sizeof(cons_t) * (size_t)c->ncell);
     compute_slopes(c, c->Ws);
     sweep_o2(c, c->Ws, &dtMax);
+#pragma omp parallel for schedule(static)
     for (i = 0; i < c->ncell; i++) {
       cst old = c->W[i];
       time_evolution(c, &c->acc[i], &old, dt);
@@ -1870,7 +1903,13 @@ int orc_step(orc_ctx* c, double dt, double* dtmax_out, orc_stats* st)
     }
   }
   /* Run::advancingProcedure step 4 (Run.cpp:593) */
-  for (i = 0; i < c->ncell; i++) relax_and_correct(c, &c->W[i], dt, st);
+#pragma omp parallel for reduction(+ : rc, ni, nf) schedule(static)
+  for (i = 0; i < c->ncell; i++) {
+    orc_stats ls = {0, 0, 0};
+    relax_and_correct(c, &c->W[i], dt, &ls);
+    rc += ls.relax_calls; ni += ls.newton_iters; nf += ls.newton_fail;
+  }
+  if (st) { st->relax_calls += rc; st->newton_iters += ni; st->newton_fail += nf; }
   if (dtmax_out) *dtmax_out = dtMax;
   /* Errors::errorMessage("solveRiemannInletTank not converged ...") ends the reference's run */
   return c->tank_fail ? -2 : 0;
@@ -1989,4 +2028,13 @@ int orc_tank_flux(int model, int n, const orc_eos* eos, double eps, const orc_ta
   out[o++] = F.uStar;
   if (dtmax) *dtmax = dtm;
   return it;
+}
+
+int orc_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }

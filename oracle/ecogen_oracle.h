@@ -91,6 +91,9 @@ int orc_step(orc_ctx* c, double dt, double* dtmax, orc_stats* st);
 int orc_run(orc_ctx* c, int max_iter, double t_end, double* t_out, double* dt_next_out, int* iter_out,
             orc_stats* st);
 
+/* Threads the sweeps use (OpenMP; OMP_NUM_THREADS), 1 without OpenMP.  Results do not depend on it. */
+int orc_threads(void);
+
 /* Point-wise kernels exposed for pinning tests. */
 double orc_eos_energy(const orc_eos* q, double eps, double rho, double p);
 double orc_eos_pressure(const orc_eos* q, double rho, double e);

@@ -102,6 +102,7 @@ def lib():
         L.orc_tank_flux.argtypes = [C.c_int, C.c_int, C.POINTER(OrcEos), C.c_double, C.POINTER(OrcTank), dp, dp, dp,
                                     dp, C.c_double, dp, dp]
         L.orc_step.restype = C.c_int
+        L.orc_threads.restype = C.c_int
         L.orc_run.restype = C.c_int
         L.orc_relax_cell.restype = C.c_int
         L.orc_relax_cell.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, C.c_int, dp, dp, dp, dp]
