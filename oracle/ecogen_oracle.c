@@ -2030,6 +2030,15 @@ int orc_tank_flux(int model, int n, const orc_eos* eos, double eps, const orc_ta
   return it;
 }
 
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 int orc_threads(void)
 {
 #ifdef _OPENMP

@@ -93,6 +93,7 @@ int orc_run(orc_ctx* c, int max_iter, double t_end, double* t_out, double* dt_ne
 
 /* Threads the sweeps use (OpenMP; OMP_NUM_THREADS), 1 without OpenMP.  Results do not depend on it. */
 int orc_threads(void);
+void orc_set_threads(int n);
 
 /* Point-wise kernels exposed for pinning tests. */
 double orc_eos_energy(const orc_eos* q, double eps, double rho, double p);

@@ -103,6 +103,7 @@ def lib():
                                     dp, C.c_double, dp, dp]
         L.orc_step.restype = C.c_int
         L.orc_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_run.restype = C.c_int
         L.orc_relax_cell.restype = C.c_int
         L.orc_relax_cell.argtypes = [C.c_int, C.POINTER(OrcEos), C.c_double, C.c_int, dp, dp, dp, dp]
