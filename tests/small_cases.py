@@ -222,6 +222,26 @@ def tank_failure_case():
                               p=[110096271.95814788] * 2, vel=[2607.606207829999, 0., 0.])])
 
 
+def tiny_tank_cases():
+    """Smallest meshes with a tank on every side: one face per side in 1D, ragged 2D planes.
+    The two-phase case runs at 1e8-3e8 Pa, where float storage of a stiffened-gas
+    liquid's energy holds its pressure (at 1e5 Pa one float rounding of the energy
+    moves p by ~gamma * pInf / p * 6e-8 ~ 2e-3)."""
+    e1 = dict(name="euler_1d_tiny_tanks", nx=2, ny=1, nz=1, lx=1.0, ly=0.1, lz=0.1, model="euler",
+              eos=[dict(IG_AIR)], order=2, limiters=("minmod",) * 4, cfl=0.8,
+              bc=("inletTank", "inletTank") + ("nonreflecting",) * 4,
+              tanks={0: dict(p0=3.e5, T0=400.), 1: dict(p0=5.e4, T0=300.)},
+              domains=[dict(kind="entire", rho=[1.0], p=[1.e5], vel=[10., 0., 0.])])
+    tk = dict(p0=3.e8, T0=176.5, alpha=[0.9, 0.1])
+    p2 = dict(name="pueq_2d_tiny_tanks", nx=3, ny=2, nz=1, lx=1.0, ly=0.7, lz=0.1, model="pueq",
+              eos=[dict(SG_WATER), dict(IG_AIR)], order=1, limiters=("none",) * 4, cfl=0.8, alpha_null=False,
+              relax_p=True, bc=("inletTank",) * 4 + ("nonreflecting",) * 2,
+              tanks={0: tk, 1: dict(tk, p0=2.5e8), 2: dict(tk, p0=1.e8), 3: dict(tk, p0=2.2e8, alpha=[0.5, 0.5])},
+              domains=[dict(kind="entire", alpha=[1. - 1e-6, 1e-6], rho=[1000., 150.], p=[2.e8, 2.e8],
+                            vel=[0., 0., 0.])])
+    return [e1, p2]
+
+
 def all_cases():
     cs = [pueq_3d_tiles(2), pueq_2d_xcd(1), pueq_2d_xcd(2), euler_sod(1), euler_sod(2), euler_2d(1), euler_2d(2), euler_2d(2, lim="superbee"), euler_3d(1),
           euler_3d(2), wall_variant(euler_2d(1)), wall_variant(euler_3d(1)),
@@ -246,4 +266,5 @@ def all_cases():
                                                  "nonreflecting")),
                        {1: dict(p0=3.e5, T0=176.5, alpha=[0.9, 0.1])}),
           tank_variant(finite_relax(ueq_2d(1), 1e-7), {1: dict(p0=3.e5, T0=300., alpha=[0.8, 0.2])})]
+    cs += tiny_tank_cases()
     return {c["name"]: c for c in cs}
