@@ -242,6 +242,20 @@ def tiny_tank_cases():
     return [e1, p2]
 
 
+def tiny_wall_cases():
+    """Smallest meshes with second-order walls: a 2-cell Sod tube closed at both ends,
+    and a 4x2 water/air mesh walled on all four sides with THINC at its interface."""
+    e1 = wall_variant(euler_sod(2, n=2), bc=("wall", "wall") + ("nonreflecting",) * 4)
+    e1["name"] = "euler_1d_tiny_walls"
+    p2 = dict(name="pueq_2d_tiny", nx=4, ny=2, nz=1, lx=1.0, ly=0.5, lz=0.1, model="pueq",
+              eos=[dict(SG_WATER), dict(IG_AIR)], order=2, limiters=("minmod",) * 4, cfl=0.8, alpha_null=False,
+              relax_p=True, bc=("nonreflecting",) * 6,
+              domains=[dict(kind="entire", alpha=[1. - 1e-6, 1e-6], rho=[1000., 1.2], p=[1.e5, 1.e5],
+                            vel=[0., 0., 0.])])
+    p2 = thinc(wall_variant(p2, bc=("wall",) * 4 + ("nonreflecting",) * 2))
+    return [e1, p2]
+
+
 def all_cases():
     cs = [pueq_3d_tiles(2), pueq_2d_xcd(1), pueq_2d_xcd(2), euler_sod(1), euler_sod(2), euler_2d(1), euler_2d(2), euler_2d(2, lim="superbee"), euler_3d(1),
           euler_3d(2), wall_variant(euler_2d(1)), wall_variant(euler_3d(1)),
@@ -266,5 +280,5 @@ def all_cases():
                                                  "nonreflecting")),
                        {1: dict(p0=3.e5, T0=176.5, alpha=[0.9, 0.1])}),
           tank_variant(finite_relax(ueq_2d(1), 1e-7), {1: dict(p0=3.e5, T0=300., alpha=[0.8, 0.2])})]
-    cs += tiny_tank_cases()
+    cs += tiny_tank_cases() + tiny_wall_cases()
     return {c["name"]: c for c in cs}
