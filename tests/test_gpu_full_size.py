@@ -1,0 +1,64 @@
+"""Size-independent properties of the step at the full 512^3 size of the
+two-phase pocket workload (needs a GPU and about 11 GB of host memory per
+downloaded state).
+
+With walls on all six sides nothing crosses the boundary: the wall flux has no
+mass, volume-fraction or energy part, only the pressure on the momentum
+(ModUEq::solveRiemannWall, src/Models/UEq/ModUEq.cpp:452-485), and the cell
+update adds each interior face's flux to one neighbour and subtracts the same
+flux from the other (Cell::timeEvolution, src/Order1/Cell.cpp:286).  The pressure relaxation
+keeps every phase's mass and the mixture energy of its cell.  So every phase's
+total mass sum_cells alpha_k rho_k and the total energy sum_cells rho E stay
+what they were, up to rounding: 1e-12 relative in FP64, 1e-6 in the FP32 form,
+whose state is stored in float (one rounding, ~6e-8, of alpha, rho and E in
+each cell that changes).
+"""
+import numpy as np
+import pytest
+
+N = 512
+STEPS = 3
+TOL = {"fp64": 1e-12, "fp32": 1e-6}
+
+
+def _totals(case, st, names):
+    """Each phase's mass and the total energy, summed over the cells (uniform cell volume)."""
+    nph = len(case["eos"])
+    mass = [st[names.index(f"alpha{k}")] * st[names.index(f"rho{k}")] for k in range(nph)]
+    rho = sum(mass)
+    return [float(np.sum(m)) for m in mass] + [float(np.sum(rho * st[names.index("E")]))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_closed_box_conserves_mass_and_energy_at_512(precision):
+    from ecogen_amd import Solver
+    from ecogen_amd.case import ueq2_pockets_3d, variable_names
+
+    case = ueq2_pockets_3d(n=N)
+    case["bc"] = ("wall",) * 6
+    case["precision"] = precision
+    names = variable_names(case)
+    print(f"[full-size] {case['name']} walls {precision}", flush=True)
+    g = Solver(case)
+    st = g.get_state()
+    before = _totals(case, st, names)
+    del st
+    dt = 1.e-15
+    for it in range(STEPS):
+        dtm, _ = g.step(dt)
+        assert np.isfinite(dtm) and dtm > 0.
+        dt = case["cfl"] * dtm
+    st = g.get_state()
+    assert np.all(np.isfinite(st))
+    # Something moved: the 5e8 Pa slab drives a wave into the box.
+    u = st[names.index("u")]
+    assert float(np.max(np.abs(u))) > 1.
+    after = _totals(case, st, names)
+    del st, u
+    g.close()
+    labels = [f"mass{k}" for k in range(len(case["eos"]))] + ["energy"]
+    for lab, b, a in zip(labels, before, after):
+        rel = abs(a - b) / abs(b)
+        print(f"[full-size] {precision} {lab}: {b:.17e} -> {a:.17e} (rel {rel:.2e})", flush=True)
+        assert rel <= TOL[precision], f"{lab}: relative change {rel:.3e}"
