@@ -4,7 +4,7 @@ Both solvers start from the same initial-condition domains, take the same
 sequence of time steps (dt0 = 1e-15, then cfl*dtMax of the oracle) and are
 compared after every step:
   * dtMax within 1e-12 relative,
-  * every persistent variable within 1e-12 relative L1 (the north-star bar;
+  * every persistent variable within 1e-12 relative L1 (the FP64 parity bar;
     the only non-shared arithmetic is the device pow() in the isentropic star
     pressures, which may differ from glibc's by an ulp).
 Cases with a THINC limiter compare with the oracle's correctly rounded exp, the
