@@ -25,7 +25,8 @@ CASES = {k: v for k, v in all_cases().items() if k in (
     "euler_1d_o1", "euler_2d_o2", "euler_3d_o1", "pueq_1d_o2", "pueq_2d_o1", "pueq_2d_o2", "pueq3_3d_o2",
     "ueq_2d_o1", "ueq_2d_o2", "ueq3_3d_o1", "ueq2_3d_o2", "pueq3_3d_tiles_o2", "pueq_2d_o1_wall",
     "pueq_2d_o2_wall", "euler_2d_o2_wall", "euler_2d_o2_tank", "pueq_2d_o1_tank", "ueq2_3d_o2_tank",
-    "euler_1d_tiny_tanks", "pueq_2d_tiny_tanks", "euler_1d_tiny_walls")}
+    "euler_1d_tiny_tanks", "pueq_2d_tiny_tanks", "euler_1d_tiny_walls", "pueq_1d_o2_thinc",
+    "ueq2_3d_o2_thinc_mix", "ueq_2d_o1_fin1e-07", "ueq_2d_o2_fin0.0001")}
 STEPS = 12
 TOL = 2e-5
 TOL_MIX = 1e-5
